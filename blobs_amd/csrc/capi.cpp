@@ -1,0 +1,205 @@
+// capi.cpp -- extern "C" entry points declared in include/blobs_gpu.h.
+// Thin: argument checks, exception containment, dispatch to bg::World.
+#include <string.h>
+
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/blobs_gpu.h"
+#include "world.hpp"
+
+struct bg_world {
+    bg::World impl;
+    bg_world(float gx, float gy, bool sh) : impl(gx, gy, sh) {}
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(bg_world* w, F&& f) {
+    if (!w) {
+        g_last_error = "null world";
+        return BG_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        int st = f(w->impl);
+        if (st != BG_OK) g_last_error = w->impl.last_error;
+        return st;
+    } catch (const std::bad_alloc&) {
+        w->impl.last_error = g_last_error = "host out of memory";
+        return BG_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        w->impl.last_error = g_last_error = e.what();
+        return BG_ERR_PANIC;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bg_world_create(float gravity_x, float gravity_y, int use_spatial_hash, bg_world** out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    try {
+        *out = new bg_world(gravity_x, gravity_y, use_spatial_hash != 0);
+    } catch (...) {
+        g_last_error = "allocation failed";
+        return BG_ERR_CAPACITY;
+    }
+    return BG_OK;
+}
+
+int bg_world_destroy(bg_world* w) {
+    delete w;
+    return BG_OK;
+}
+
+int bg_world_reset(bg_world* w) {
+    return guarded(w, [&](bg::World& x) { return x.reset(); });
+}
+
+int bg_get_params(bg_world* w, bg_params* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.get_params(out); });
+}
+
+int bg_set_params(bg_world* w, const bg_params* p) {
+    if (!p) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.set_params(p); });
+}
+
+int bg_insert_rbd(bg_world* w, const bg_rigid_body* body, bg_handle* out) {
+    if (!body) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.insert_rbd(*body, out); });
+}
+
+int bg_insert_collider_with_parent(bg_world* w, const bg_collider* col, bg_handle parent, bg_handle* out) {
+    if (!col) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.insert_collider_with_parent(*col, parent, out); });
+}
+
+int bg_insert_rbds(bg_world* w, uint32_t n, const bg_body_batch* b, bg_handle* out) {
+    if (!b) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.insert_rbds(n, *b, out); });
+}
+
+int bg_insert_colliders_with_parents(bg_world* w, uint32_t n, const bg_collider_batch* c, const bg_handle* parents,
+                                     bg_handle* out) {
+    if (!c) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.insert_colliders(n, *c, parents, out); });
+}
+
+int bg_create_fixed_joint(bg_world* w, bg_handle a, bg_handle b, const float anchor_a[2], const float anchor_b[2],
+                          bg_handle* out) {
+    if (!anchor_a || !anchor_b) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.create_fixed_joint(a, b, anchor_a, anchor_b, false, 0.0f, out); });
+}
+
+int bg_create_fixed_joint_with_distance(bg_world* w, bg_handle a, bg_handle b, const float anchor_a[2],
+                                        const float anchor_b[2], float distance, bg_handle* out) {
+    if (!anchor_a || !anchor_b) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w,
+                   [&](bg::World& x) { return x.create_fixed_joint(a, b, anchor_a, anchor_b, true, distance, out); });
+}
+
+int bg_insert_spring(bg_world* w, const bg_spring* s, bg_handle* out) {
+    if (!s) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.insert_spring(*s, out); });
+}
+
+int bg_push_constraint(bg_world* w, float x_, float y_, float radius) {
+    return guarded(w, [&](bg::World& x) { return x.push_constraint(x_, y_, radius); });
+}
+
+int bg_remove_rbd(bg_world* w, bg_handle h) {
+    return guarded(w, [&](bg::World& x) { return x.remove_rbd(h); });
+}
+
+int bg_remove_col(bg_world* w, bg_handle h) {
+    return guarded(w, [&](bg::World& x) { return x.remove_col(h); });
+}
+
+int bg_step(bg_world* w, double delta, uint32_t n_steps) {
+    return guarded(w, [&](bg::World& x) { return x.step(delta, n_steps); });
+}
+
+int bg_fixed_step(bg_world* w, double frame_time) {
+    return guarded(w, [&](bg::World& x) { return x.fixed_step(frame_time); });
+}
+
+int bg_synchronize(bg_world* w) {
+    return guarded(w, [&](bg::World& x) { return x.synchronize(); });
+}
+
+int bg_rbd_count(bg_world* w, uint32_t* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) {
+        *out = x.rbd_count();
+        return BG_OK;
+    });
+}
+
+int bg_get_rbd(bg_world* w, bg_handle h, bg_rigid_body* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.get_rbd(h, out); });
+}
+
+int bg_set_rbd(bg_world* w, bg_handle h, const bg_rigid_body* in) {
+    if (!in) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.set_rbd(h, in); });
+}
+
+int bg_get_col(bg_world* w, bg_handle h, bg_collider* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.get_col(h, out); });
+}
+
+int bg_get_bodies(bg_world* w, float* position, float* position_old, float* velocity, float* rotation,
+                  float* angular_velocity, bg_handle* handles) {
+    return guarded(w, [&](bg::World& x) {
+        return x.get_bodies(position, position_old, velocity, rotation, angular_velocity, handles);
+    });
+}
+
+int bg_get_collider_positions(bg_world* w, float* absolute_translation, uint32_t* count) {
+    return guarded(w, [&](bg::World& x) { return x.get_collider_positions(absolute_translation, count); });
+}
+
+int bg_contact_count(bg_world* w, uint64_t* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.contact_count(out); });
+}
+
+int bg_last_error(bg_world* w, char* buf, size_t len) {
+    if (!buf || len == 0) return BG_ERR_INVALID_ARGUMENT;
+    const std::string& s = w ? w->impl.last_error : g_last_error;
+    size_t n = s.size() < len - 1 ? s.size() : len - 1;
+    memcpy(buf, s.data(), n);
+    buf[n] = 0;
+    return BG_OK;
+}
+
+int bg_set_broadphase(bg_world* w, float cell_scale, uint32_t rebuild_interval) {
+    return guarded(w, [&](bg::World& x) { return x.set_broadphase(cell_scale, rebuild_interval); });
+}
+
+int bg_step_path(bg_world* w, uint32_t* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) {
+        *out = (uint32_t)x.path();
+        return BG_OK;
+    });
+}
+
+int bg_set_profiling(bg_world* w, int enable) {
+    return guarded(w, [&](bg::World& x) { return x.set_profiling(enable != 0); });
+}
+
+int bg_stage_stats(bg_world* w, uint32_t stage, uint64_t* launches, double* total_ms) {
+    return guarded(w, [&](bg::World& x) { return x.stage_stats(stage, launches, total_ms); });
+}
+
+}  // extern "C"

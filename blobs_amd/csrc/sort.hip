@@ -1,0 +1,249 @@
+// sort.hip -- exclusive scan and stable LSD radix sort for gfx950.
+//
+// Scan: one launch, decoupled look-back (tiles take a ticket in launch order,
+// publish their aggregate, then walk back over predecessors until an
+// inclusive prefix is found).  Radix sort: 8-bit digits; per pass a digit
+// histogram per 4096-item tile, the scan above over the digit-major matrix,
+// and a scatter that ranks items with wave64 ballots (peer masks from 8
+// ballots, popcount of lower lanes), orders the tile in LDS and writes
+// digit runs out contiguously.
+#include "common.hpp"
+#include "sort.hpp"
+
+namespace bg {
+
+namespace {
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagIncl = 2ull << 62;
+
+__global__ void k_fill_u64(uint64_t* p, uint32_t n, uint64_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                                 uint64_t* status, uint32_t ntiles) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x;
+    const int lane = tid & (BG_WAVE - 1);
+    const int wid = tid / BG_WAVE;
+    if (tid == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(&status[ntiles]), 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * kScanTile + tid * kSortItems;
+    uint32_t v[kSortItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        uint32_t i = base + k;
+        v[k] = (i < n) ? in[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t incl = wave_incl_scan_u32(sum);
+    if (lane == BG_WAVE - 1) s_wave[wid] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / BG_WAVE; ++w) {
+        if (w < wid) wave_off += s_wave[w];
+        total += s_wave[w];
+    }
+    if (tid == 0) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], kFlagIncl | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], kFlagAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int32_t j = (int32_t)tile - 1;
+            while (j >= 0) {
+                uint64_t st = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t flag = st & (3ull << 62);
+                if (flag == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)st;
+                if (flag == kFlagIncl) break;
+                --j;
+            }
+            __hip_atomic_store(&status[tile], kFlagIncl | (uint32_t)(excl + total), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = excl;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wave_off + (incl - sum);
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        uint32_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
+                                                             uint32_t* hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        uint32_t i = base + k * kSortThreads + tid;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[tid * nblocks + blockIdx.x] = h[tid];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
+                                                                const uint32_t* __restrict__ vals_in,
+                                                                uint32_t* __restrict__ keys_out,
+                                                                uint32_t* __restrict__ vals_out, uint32_t n,
+                                                                int shift, const uint32_t* __restrict__ offsets,
+                                                                uint32_t nblocks) {
+    constexpr int kWaves = kSortThreads / BG_WAVE;
+    __shared__ uint32_t wcount[kWaves][256];
+    __shared__ uint32_t dbo[256];
+    __shared__ uint32_t s_dwave[kWaves];
+    __shared__ uint32_t stage_k[kSortTile];
+    __shared__ uint32_t stage_v[kSortTile];
+    const int tid = threadIdx.x;
+    const int lane = tid & (BG_WAVE - 1);
+    const int wid = tid / BG_WAVE;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wcount[w][tid] = 0;
+    __syncthreads();
+
+    const uint32_t tile_base = blockIdx.x * kSortTile;
+    const uint32_t sub_base = tile_base + wid * (kSortTile / kWaves);
+    const uint64_t lt = lanemask_lt();
+    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        uint32_t i = sub_base + r * BG_WAVE + lane;
+        bool valid = i < n;
+        key[r] = valid ? keys_in[i] : 0u;
+        val[r] = valid ? vals_in[i] : 0u;
+        uint32_t d = (key[r] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bits = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bits : ~bits;
+        }
+        uint32_t below = (uint32_t)__popcll(peers & lt);
+        uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t prior = valid ? wcount[wid][d] : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) wcount[wid][d] = prior + cnt;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        rank[r] = prior + below;
+    }
+    __syncthreads();
+    // per digit: exclusive prefix over waves, then over digits for the block
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        uint32_t c = wcount[w][tid];
+        wcount[w][tid] = tot;
+        tot += c;
+    }
+    uint32_t incl = wave_incl_scan_u32(tot);
+    if (lane == BG_WAVE - 1) s_dwave[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+        if (w < wid) woff += s_dwave[w];
+    dbo[tid] = woff + incl - tot;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        uint32_t i = sub_base + r * BG_WAVE + lane;
+        if (i < n) {
+            uint32_t d = (key[r] >> shift) & 255u;
+            uint32_t p = dbo[d] + wcount[wid][d] + rank[r];
+            stage_k[p] = key[r];
+            stage_v[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t count = (n - tile_base) < (uint32_t)kSortTile ? (n - tile_base) : (uint32_t)kSortTile;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        uint32_t p = k * kSortThreads + tid;
+        if (p < count) {
+            uint32_t kk = stage_k[p];
+            uint32_t d = (kk >> shift) & 255u;
+            uint32_t g = offsets[d * nblocks + blockIdx.x] + (p - dbo[d]);
+            if (g < n) {  // always true for a consistent histogram/scan
+                keys_out[g] = kk;
+                vals_out[g] = stage_v[p];
+            }
+        }
+    }
+}
+
+inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+hipError_t sort_temp_reserve(SortTemp& t, uint32_t max_items) {
+    if (max_items <= t.max_items && t.hist) return hipSuccess;
+    sort_temp_free(t);
+    uint32_t mi = max_items < 4096 ? 4096 : max_items;
+    t.max_items = mi;
+    t.max_blocks = div_up(mi, kSortTile);
+    // the scan runs over 256 * max_blocks digit counters or over max_items
+    uint32_t scan_items = 256u * t.max_blocks;
+    if (scan_items < mi + 1) scan_items = mi + 1;
+    t.max_scan_tiles = div_up(scan_items, kScanTile);
+    hipError_t e = hipMalloc(&t.hist, sizeof(uint32_t) * 256u * t.max_blocks);
+    if (e != hipSuccess) return e;
+    return hipMalloc(&t.scan_status, sizeof(uint64_t) * (t.max_scan_tiles + 1));
+}
+
+void sort_temp_free(SortTemp& t) {
+    if (t.hist) (void)hipFree(t.hist);
+    if (t.scan_status) (void)hipFree(t.scan_status);
+    t = SortTemp{};
+}
+
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t ntiles = div_up(n, kScanTile);
+    if (ntiles > t.max_scan_tiles) return hipErrorInvalidValue;
+    k_fill_u64<<<div_up(ntiles + 1, 256), 256, 0, s>>>(t.scan_status, ntiles + 1, 0ull);
+    k_scan_exclusive<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.scan_status, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t radix_sort_pairs(uint32_t** keys, uint32_t** vals, uint32_t** keys_tmp, uint32_t** vals_tmp, uint32_t n,
+                            int bits, SortTemp& t, hipStream_t s) {
+    if (n <= 1 || bits <= 0) return hipSuccess;
+    if (n > t.max_items) return hipErrorInvalidValue;
+    uint32_t nblocks = div_up(n, kSortTile);
+    for (int shift = 0; shift < bits; shift += 8) {
+        k_radix_hist<<<nblocks, kSortThreads, 0, s>>>(*keys, n, shift, t.hist, nblocks);
+        hipError_t e = exclusive_scan_u32(t.hist, t.hist, 256u * nblocks, t, s);
+        if (e != hipSuccess) return e;
+        k_radix_scatter<<<nblocks, kSortThreads, 0, s>>>(*keys, *vals, *keys_tmp, *vals_tmp, n, shift, t.hist,
+                                                         nblocks);
+        uint32_t* tk = *keys;
+        *keys = *keys_tmp;
+        *keys_tmp = tk;
+        uint32_t* tv = *vals;
+        *vals = *vals_tmp;
+        *vals_tmp = tv;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bg

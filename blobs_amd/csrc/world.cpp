@@ -1,0 +1,1068 @@
+// world.cpp -- host side of the `Physics` drop-in (see world.hpp).
+#include "world.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+
+namespace bg {
+
+namespace {
+
+constexpr uint32_t kAll = 0xFFFFFFFFu;
+
+bool same_handle(bg_handle a, bg_handle b) { return a == b; }
+
+float col_mass(const bg_collider& c) { return c.has_mass_override ? c.mass_override : c.radius * 2.0f; }
+
+// Collider::inertia, collider.rs:44-50.
+float col_inertia(const bg_collider& c) {
+    float m = col_mass(c);
+    float ox = c.offset_translation[0], oy = c.offset_translation[1];
+    float d = sqrtf(ox * ox + oy * oy);
+    float inertia = 0.5f * m * (c.radius * c.radius);
+    return inertia + m * (d * d);
+}
+
+template <class T>
+hipError_t dmalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <class T>
+hipError_t h2d(T* dst, const std::vector<T>& src, hipStream_t s) {
+    if (src.empty()) return hipSuccess;
+    return hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+template <class T>
+hipError_t d2h(std::vector<T>& dst, const T* src, size_t n, hipStream_t s) {
+    dst.resize(n);
+    if (n == 0) return hipSuccess;
+    return hipMemcpyAsync(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, s);
+}
+
+uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) return fail(BG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+}  // namespace
+
+World::World(float gx, float gy, bool use_spatial_hash) {
+    // Physics::new defaults, physics.rs:42-68
+    params_.gravity[0] = gx;
+    params_.gravity[1] = gy;
+    params_.substeps = 8;
+    params_.joint_iterations = 4;
+    params_.collisions_enabled = 1;
+    params_.use_spatial_hash = use_spatial_hash ? 1u : 0u;
+    params_.accumulator = 0.0;
+    params_.time = 0.0;
+    params_.old_dt = 1.0f;
+}
+
+World::~World() {
+    if (d_.stream) (void)hipStreamSynchronize(d_.stream);
+    for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+    for (auto& v : pending_)
+        for (auto& pr : v) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    free_device();
+}
+
+int World::set_profiling(bool enable) {
+    if (!enable && profiling_) {
+        int st = collect_stage_times();
+        if (st) return st;
+    }
+    profiling_ = enable;
+    for (int k = 0; k < BG_STAGE_COUNT; ++k) {
+        stage_launches_[k] = 0;
+        stage_ms_[k] = 0.0;
+    }
+    return BG_OK;
+}
+
+int World::stage_stats(uint32_t stage, uint64_t* launches, double* total_ms) {
+    if (stage >= BG_STAGE_COUNT) return fail(BG_ERR_INVALID_ARGUMENT, "unknown stage");
+    int st = collect_stage_times();
+    if (st) return st;
+    if (launches) *launches = stage_launches_[stage];
+    if (total_ms) *total_ms = stage_ms_[stage];
+    return BG_OK;
+}
+
+int World::stage_begin(uint32_t stage) {
+    if (!profiling_) return BG_OK;
+    hipEvent_t e;
+    if (!event_pool_.empty()) {
+        e = event_pool_.back();
+        event_pool_.pop_back();
+    } else {
+        HIP_TRY(hipEventCreate(&e));
+    }
+    HIP_TRY(hipEventRecord(e, d_.stream));
+    open_[stage] = e;
+    return BG_OK;
+}
+
+int World::stage_end(uint32_t stage) {
+    if (!profiling_ || !open_[stage]) return BG_OK;
+    hipEvent_t e;
+    if (!event_pool_.empty()) {
+        e = event_pool_.back();
+        event_pool_.pop_back();
+    } else {
+        HIP_TRY(hipEventCreate(&e));
+    }
+    HIP_TRY(hipEventRecord(e, d_.stream));
+    pending_[stage].push_back({open_[stage], e});
+    open_[stage] = nullptr;
+    return BG_OK;
+}
+
+int World::collect_stage_times() {
+    for (int k = 0; k < BG_STAGE_COUNT; ++k) {
+        for (auto& pr : pending_[k]) {
+            HIP_TRY(hipEventSynchronize(pr.second));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+            stage_ms_[k] += ms;
+            stage_launches_[k] += 1;
+            event_pool_.push_back(pr.first);
+            event_pool_.push_back(pr.second);
+        }
+        pending_[k].clear();
+    }
+    return BG_OK;
+}
+
+// RigidBody::update_mass_and_inertia, rigid_body.rs:96-128.
+void World::update_mass_and_inertia(HostBody& b) {
+    float m = 0.0f, I = 0.0f, wx = 0.0f, wy = 0.0f;
+    for (bg_handle h : b.colliders) {
+        const HostCollider* c = colliders_.get(h);
+        if (!c) continue;  // the reference prints "Collider .. not found" and skips
+        if (c->f.is_sensor) continue;
+        float cm = col_mass(c->f);
+        m = m + cm;
+        I = I + col_inertia(c->f);
+        wx = wx + c->f.offset_translation[0] * cm;
+        wy = wy + c->f.offset_translation[1] * cm;
+    }
+    if (m == 0.0f) m = 1.0f;
+    if (I == 0.0f) I = 1.0f;
+    b.f.calculated_mass = m;
+    b.f.inertia = I;
+    b.f.center_of_mass[0] = wx / m;
+    b.f.center_of_mass[1] = wy / m;
+}
+
+int World::insert_rbd(const bg_rigid_body& b, bg_handle* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostBody hb;
+    hb.f = b;
+    bg_handle h = bodies_.insert(hb);
+    if (out) *out = h;
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::insert_collider_with_parent(const bg_collider& c, bg_handle parent, bg_handle* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostCollider hc;
+    hc.f = c;
+    hc.f.parent = parent;  // collider.rs:172
+    hc.f.has_parent = 1;
+    bg_handle h = colliders_.insert(hc);
+    if (out) *out = h;
+    mark_dirty();
+    HostBody* b = bodies_.get(parent);
+    if (!b) return fail(BG_ERR_PANIC, "parent rigid body must exist when inserting collider");  // physics.rs:142
+    b->colliders.push_back(h);  // collider.rs:179-181
+    b->colliders.push_back(h);  // physics.rs:144
+    update_mass_and_inertia(*b);  // physics.rs:146
+    return BG_OK;
+}
+
+int World::insert_rbds(uint32_t n, const bg_body_batch& bb, bg_handle* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    if (!bb.position) return fail(BG_ERR_INVALID_ARGUMENT, "position array is required");
+    bodies_.reserve(bodies_.capacity_slots() + n);
+    for (uint32_t i = 0; i < n; ++i) {
+        // RigidBodyBuilder::new() defaults (rigid_body.rs:303-318), then .position(p) (:320-324)
+        HostBody hb;
+        bg_rigid_body& f = hb.f;
+        memset(&f, 0, sizeof(f));
+        f.position[0] = f.position_old[0] = bb.position[2 * i];
+        f.position[1] = f.position_old[1] = bb.position[2 * i + 1];
+        f.gravity_mod = bb.gravity_mod ? bb.gravity_mod[i] : 1.0f;
+        f.rotation = bb.rotation ? bb.rotation[i] : 0.0f;
+        f.scale[0] = f.scale[1] = 1.0f;
+        if (bb.velocity_request && (!bb.has_velocity_request || bb.has_velocity_request[i])) {
+            f.velocity_request[0] = bb.velocity_request[2 * i];
+            f.velocity_request[1] = bb.velocity_request[2 * i + 1];
+            f.has_velocity_request = 1;
+        }
+        f.body_type = bb.body_type ? bb.body_type[i] : BG_DYNAMIC;
+        f.calculated_mass = 1.0f;  // build(), rigid_body.rs:385-388
+        f.inertia = 1.0f;
+        bg_handle h = bodies_.insert(hb);
+        if (out) out[i] = h;
+    }
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::insert_colliders(uint32_t n, const bg_collider_batch& cb, const bg_handle* parents, bg_handle* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    if (!cb.radius || !parents) return fail(BG_ERR_INVALID_ARGUMENT, "radius and parents are required");
+    colliders_.reserve(colliders_.capacity_slots() + n);
+    for (uint32_t i = 0; i < n; ++i) {
+        // ColliderBuilder::new() defaults (collider.rs:212-224)
+        bg_collider c;
+        memset(&c, 0, sizeof(c));
+        c.offset_matrix[0] = c.offset_matrix[3] = 1.0f;
+        c.absolute_matrix[0] = c.absolute_matrix[3] = 1.0f;
+        if (cb.offset_translation) {
+            c.offset_translation[0] = cb.offset_translation[2 * i];
+            c.offset_translation[1] = cb.offset_translation[2 * i + 1];
+        }
+        if (cb.absolute_translation) {
+            c.absolute_translation[0] = cb.absolute_translation[2 * i];
+            c.absolute_translation[1] = cb.absolute_translation[2 * i + 1];
+        }
+        c.radius = cb.radius[i];
+        if (cb.has_mass_override && cb.has_mass_override[i]) {
+            c.has_mass_override = 1;
+            c.mass_override = cb.mass_override ? cb.mass_override[i] : 0.0f;
+        }
+        c.is_sensor = cb.is_sensor ? cb.is_sensor[i] : 0;
+        c.memberships = cb.memberships ? cb.memberships[i] : kAll;
+        c.filter = cb.filter ? cb.filter[i] : kAll;
+        int st = insert_collider_with_parent(c, parents[i], out ? &out[i] : nullptr);
+        if (st != BG_OK) return st;
+    }
+    return BG_OK;
+}
+
+int World::create_fixed_joint(bg_handle a, bg_handle b, const float* aa, const float* ab, bool with_distance,
+                              float distance, bg_handle* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostBody* ba = bodies_.get(a);
+    HostBody* bb = bodies_.get(b);
+    // get2_mut(..).zip().unwrap() panics on a missing body or a == b (physics.rs:191-196)
+    if (!ba || !bb || same_handle(a, b)) return fail(BG_ERR_PANIC, "create_fixed_joint: bodies must exist and differ");
+    if (!with_distance) {
+        // (rbd_a.position + anchor_a - rbd_b.position - anchor_b).length(), physics.rs:198
+        float x = ((ba->f.position[0] + aa[0]) - bb->f.position[0]) - ab[0];
+        float y = ((ba->f.position[1] + aa[1]) - bb->f.position[1]) - ab[1];
+        distance = sqrtf(x * x + y * y);
+    }
+    bg_fixed_joint j;
+    j.rigid_body_a = a;
+    j.rigid_body_b = b;
+    j.anchor_a[0] = aa[0];
+    j.anchor_a[1] = aa[1];
+    j.anchor_b[0] = ab[0];
+    j.anchor_b[1] = ab[1];
+    j.distance = distance;
+    j.target_angle = bb->f.rotation - ba->f.rotation;  // physics.rs:230
+    bg_handle h = joints_.insert(j);
+    ba->joints.push_back(h);
+    bb->joints.push_back(h);
+    if (out) *out = h;
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::insert_spring(const bg_spring& s, bg_handle* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    bg_handle h = springs_.insert(s);
+    if (out) *out = h;
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::push_constraint(float x, float y, float r) {
+    if (sync_host()) return BG_ERR_HIP;
+    constraints_.push_back({x, y, r});
+    mark_dirty();
+    return BG_OK;
+}
+
+// Physics::remove_rbd, physics.rs:163-172.
+int World::remove_rbd(bg_handle h) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostBody* b = bodies_.get(h);
+    if (b) {
+        for (bg_handle c : b->colliders) colliders_.remove(c);  // remove_ignoring_parent
+    }
+    bodies_.remove(h);
+    mark_dirty();
+    return BG_OK;
+}
+
+// ColliderSet::remove, collider.rs:134-164.
+int World::remove_col(bg_handle h) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostCollider* c = colliders_.get(h);
+    if (c && c->f.has_parent) {
+        bg_handle parent = c->f.parent;
+        HostBody* b = bodies_.get(parent);
+        bool remove_body = false;
+        if (b) {
+            b->colliders.erase(std::remove(b->colliders.begin(), b->colliders.end(), h), b->colliders.end());
+            update_mass_and_inertia(*b);
+            if (b->colliders.empty()) remove_body = true;
+        }
+        if (remove_body) bodies_.remove(parent);
+    }
+    colliders_.remove(h);
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::reset() {
+    if (sync_host()) return BG_ERR_HIP;
+    bodies_.clear();
+    colliders_.clear();
+    joints_.clear();
+    springs_.clear();
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::get_params(bg_params* p) {
+    if (host_stale_) {
+        // old_dt lives on the device between steps
+        if (d_.accum) {
+            StepAccum acc;
+            HIP_TRY(hipMemcpyAsync(&acc, d_.accum, sizeof(acc), hipMemcpyDeviceToHost, d_.stream));
+            HIP_TRY(hipStreamSynchronize(d_.stream));
+            params_.old_dt = acc.old_dt;
+        }
+    }
+    *p = params_;
+    return BG_OK;
+}
+
+int World::set_params(const bg_params* p) {
+    float old = params_.old_dt;
+    params_ = *p;
+    if (d_.accum && !dev_dirty_ && __builtin_memcmp(&old, &p->old_dt, 4) != 0) {
+        HIP_TRY(hipMemcpyAsync(&d_.accum->old_dt, &params_.old_dt, sizeof(float), hipMemcpyHostToDevice, d_.stream));
+        HIP_TRY(hipStreamSynchronize(d_.stream));
+    }
+    return BG_OK;
+}
+
+int World::get_rbd(bg_handle h, bg_rigid_body* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostBody* b = bodies_.get(h);
+    if (!b) return fail(BG_ERR_NOT_FOUND, "rigid body not found");
+    *out = b->f;
+    return BG_OK;
+}
+
+int World::set_rbd(bg_handle h, const bg_rigid_body* in) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostBody* b = bodies_.get(h);
+    if (!b) return fail(BG_ERR_NOT_FOUND, "rigid body not found");
+    b->f = *in;
+    mark_dirty();
+    return BG_OK;
+}
+
+int World::get_col(bg_handle h, bg_collider* out) {
+    if (sync_host()) return BG_ERR_HIP;
+    HostCollider* c = colliders_.get(h);
+    if (!c) return fail(BG_ERR_NOT_FOUND, "collider not found");
+    *out = c->f;
+    return BG_OK;
+}
+
+int World::get_bodies(float* pos, float* pos_old, float* vel, float* rot, float* omega, bg_handle* handles) {
+    if (sync_host()) return BG_ERR_HIP;
+    uint32_t k = 0;
+    for (uint32_t s = 0; s < bodies_.capacity_slots(); ++s) {
+        if (!bodies_.occupied(s)) continue;
+        const bg_rigid_body& f = bodies_.at_slot(s).f;
+        if (pos) {
+            pos[2 * k] = f.position[0];
+            pos[2 * k + 1] = f.position[1];
+        }
+        if (pos_old) {
+            pos_old[2 * k] = f.position_old[0];
+            pos_old[2 * k + 1] = f.position_old[1];
+        }
+        if (vel) {
+            vel[2 * k] = f.calculated_velocity[0];
+            vel[2 * k + 1] = f.calculated_velocity[1];
+        }
+        if (rot) rot[k] = f.rotation;
+        if (omega) omega[k] = f.angular_velocity;
+        if (handles) handles[k] = bodies_.handle_at(s);
+        ++k;
+    }
+    return BG_OK;
+}
+
+int World::get_collider_positions(float* abs, uint32_t* count) {
+    if (sync_host()) return BG_ERR_HIP;
+    uint32_t k = 0;
+    for (uint32_t s = 0; s < colliders_.capacity_slots(); ++s) {
+        if (!colliders_.occupied(s)) continue;
+        if (abs) {
+            abs[2 * k] = colliders_.at_slot(s).f.absolute_translation[0];
+            abs[2 * k + 1] = colliders_.at_slot(s).f.absolute_translation[1];
+        }
+        ++k;
+    }
+    if (count) *count = k;
+    return BG_OK;
+}
+
+int World::contact_count(uint64_t* out) {
+    uint64_t dev = 0;
+    if (d_.accum && path_ != StepPath::None) {
+        StepAccum acc;
+        HIP_TRY(hipMemcpyAsync(&acc, d_.accum, sizeof(acc), hipMemcpyDeviceToHost, d_.stream));
+        HIP_TRY(hipStreamSynchronize(d_.stream));
+        dev = acc.contacts;
+    }
+    *out = contact_base_ + dev;
+    return BG_OK;
+}
+
+int World::set_broadphase(float cell_scale, uint32_t rebuild_interval) {
+    if (!(cell_scale >= 0.25f) || rebuild_interval == 0)
+        return fail(BG_ERR_INVALID_ARGUMENT, "cell_scale must be >= 0.25 and rebuild_interval >= 1");
+    cell_scale_ = cell_scale;
+    rebuild_interval_ = rebuild_interval;
+    need_rebuild_ = true;
+    return BG_OK;
+}
+
+// ------------------------------------------------------------------ device
+
+int World::ensure_device() {
+    if (d_.stream) return BG_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(BG_ERR_NO_DEVICE, "no HIP device available: the GPU step has no CPU fallback");
+    HIP_TRY(hipGetDevice(&d_.device));
+    HIP_TRY(hipStreamCreateWithFlags(&d_.stream, hipStreamNonBlocking));
+    return BG_OK;
+}
+
+void World::free_arrays() {
+    DeviceWorld& d = d_;
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
+    for (BodyArrays* b : {&d.b, &d.b_alt}) {
+        dfree(b->pos), dfree(b->pos_old), dfree(b->vel), dfree(b->acc), dfree(b->vreq);
+        dfree(b->rot), dfree(b->omega), dfree(b->torque), dfree(b->inertia), dfree(b->mass), dfree(b->gmod);
+        dfree(b->bflags), dfree(b->aid);
+    }
+    for (ColliderArrays* c : {&d.c, &d.c_alt}) {
+        dfree(c->abs), dfree(c->off), dfree(c->radius), dfree(c->parent), dfree(c->groups), dfree(c->cflags);
+    }
+    dfree(d.abs_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
+    dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
+        dfree(d.bs_list);
+    dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
+    dfree(d.jc_start), dfree(d.jc_list), dfree(d.con_pos), dfree(d.con_r);
+    dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.slot_of);
+    dfree(d.grid), dfree(d.accum);
+    sort_temp_free(d.sort);
+    hipStream_t s = d.stream;
+    int dev = d.device;
+    d = DeviceWorld{};
+    d.stream = s;
+    d.device = dev;
+}
+
+void World::free_device() {
+    free_arrays();
+    if (d_.stream) (void)hipStreamDestroy(d_.stream);
+    d_.stream = nullptr;
+}
+
+// Flatten the arenas into dense SoA (slot order without holes = the
+// reference's iteration order) and copy them to the device.
+int World::upload() {
+    if (d_.stream) HIP_TRY(hipStreamSynchronize(d_.stream));
+    // keep the contact counter across re-uploads
+    if (d_.accum && path_ != StepPath::None) {
+        StepAccum acc;
+        HIP_TRY(hipMemcpy(&acc, d_.accum, sizeof(acc), hipMemcpyDeviceToHost));
+        contact_base_ += acc.contacts;
+    }
+    free_arrays();
+
+    structure_error_ = false;
+    structure_msg_.clear();
+
+    // dense maps
+    const uint32_t nb = bodies_.len(), nc = colliders_.len();
+    body_slot_of_dense_.clear();
+    col_slot_of_dense_.clear();
+    body_dense_of_slot_.assign(bodies_.capacity_slots(), -1);
+    col_dense_of_slot_.assign(colliders_.capacity_slots(), -1);
+    for (uint32_t s = 0; s < bodies_.capacity_slots(); ++s)
+        if (bodies_.occupied(s)) {
+            body_dense_of_slot_[s] = (int32_t)body_slot_of_dense_.size();
+            body_slot_of_dense_.push_back(s);
+        }
+    for (uint32_t s = 0; s < colliders_.capacity_slots(); ++s)
+        if (colliders_.occupied(s)) {
+            col_dense_of_slot_[s] = (int32_t)col_slot_of_dense_.size();
+            col_slot_of_dense_.push_back(s);
+        }
+    auto body_dense = [&](bg_handle h) -> int32_t {
+        if (!bodies_.get(h)) return -1;
+        return body_dense_of_slot_[Arena<HostBody>::slot_of(h)];
+    };
+    auto col_dense = [&](bg_handle h) -> int32_t {
+        if (!colliders_.get(h)) return -1;
+        return col_dense_of_slot_[Arena<HostCollider>::slot_of(h)];
+    };
+
+    // bodies
+    std::vector<float2> pos(nb), pos_old(nb), vel(nb), acc(nb), vreq(nb);
+    std::vector<float> rot(nb), omega(nb), torque(nb), inertia(nb), mass(nb), gmod(nb);
+    std::vector<uint32_t> bflags(nb), aid(nb);
+    std::vector<uint32_t> bc_start(nb + 1, 0), bc_list;
+    bc_list.reserve(nc);
+    first_dyn_ = -1;
+    for (uint32_t i = 0; i < nb; ++i) {
+        const HostBody& hb = bodies_.at_slot(body_slot_of_dense_[i]);
+        const bg_rigid_body& f = hb.f;
+        pos[i] = make_float2(f.position[0], f.position[1]);
+        pos_old[i] = make_float2(f.position_old[0], f.position_old[1]);
+        vel[i] = make_float2(f.calculated_velocity[0], f.calculated_velocity[1]);
+        acc[i] = make_float2(f.acceleration[0], f.acceleration[1]);
+        vreq[i] = make_float2(f.velocity_request[0], f.velocity_request[1]);
+        rot[i] = f.rotation;
+        omega[i] = f.angular_velocity;
+        torque[i] = f.torque;
+        inertia[i] = f.inertia;
+        mass[i] = f.calculated_mass;
+        gmod[i] = f.gravity_mod;
+        uint32_t fl = 0;
+        if (f.body_type == BG_STATIC) fl |= BF_STATIC;
+        if (f.has_velocity_request) fl |= BF_HAS_VREQ;
+        if (f.inertia > 0.0f && isfinite(f.inertia)) fl |= BF_INERTIA_POS;
+        bflags[i] = fl;
+        aid[i] = i;
+        if (first_dyn_ < 0 && f.body_type != BG_STATIC) first_dyn_ = (int32_t)i;
+        // colliders of this body, first occurrence order, duplicates dropped
+        for (bg_handle ch : hb.colliders) {
+            int32_t cd = col_dense(ch);
+            if (cd < 0) continue;
+            bool dup = false;
+            for (uint32_t k = bc_start[i]; k < bc_list.size(); ++k)
+                if (bc_list[k] == (uint32_t)cd) dup = true;
+            if (!dup) bc_list.push_back((uint32_t)cd);
+        }
+        bc_start[i + 1] = (uint32_t)bc_list.size();
+    }
+
+    // colliders
+    std::vector<float2> cabs(nc), coff(nc);
+    std::vector<float> crad(nc);
+    std::vector<int32_t> cpar(nc);
+    std::vector<uint2> cgrp(nc);
+    std::vector<uint32_t> cfl(nc);
+    rmax_ = 0.0f;
+    groups_ = false;
+    sensors_ = false;
+    zero_off_ = true;
+    float minx = INFINITY, miny = INFINITY, maxx = -INFINITY, maxy = -INFINITY;
+    for (uint32_t i = 0; i < nc; ++i) {
+        const bg_collider& f = colliders_.at_slot(col_slot_of_dense_[i]).f;
+        cabs[i] = make_float2(f.absolute_translation[0], f.absolute_translation[1]);
+        coff[i] = make_float2(f.offset_translation[0], f.offset_translation[1]);
+        uint32_t ox, oy;
+        memcpy(&ox, &f.offset_translation[0], 4);
+        memcpy(&oy, &f.offset_translation[1], 4);
+        if (ox != 0u || oy != 0u) zero_off_ = false;
+        crad[i] = f.radius;
+        if (!(f.radius <= rmax_)) rmax_ = f.radius;  // NaN radius propagates to a full scan
+        cpar[i] = f.has_parent ? body_dense(f.parent) : -1;
+        cgrp[i] = make_uint2(f.memberships, f.filter);
+        if (f.memberships != kAll || f.filter != kAll) groups_ = true;
+        cfl[i] = f.is_sensor ? 1u : 0u;
+        if (f.is_sensor) sensors_ = true;
+        float x = cabs[i].x, y = cabs[i].y;
+        if (isfinite(x) && isfinite(y)) {
+            minx = fminf(minx, x);
+            maxx = fmaxf(maxx, x);
+            miny = fminf(miny, y);
+            maxy = fmaxf(maxy, y);
+        }
+    }
+
+    // springs (arena order) and body -> spring lists
+    const uint32_t ns = springs_.len();
+    std::vector<int32_t> sp_a, sp_b;
+    std::vector<float> sp_rest, sp_k, sp_c;
+    std::vector<uint32_t> bs_count(nb + 1, 0), bs_start(nb + 1, 0), bs_list;
+    for (uint32_t s = 0; s < springs_.capacity_slots(); ++s) {
+        if (!springs_.occupied(s)) continue;
+        const bg_spring& sp = springs_.at_slot(s);
+        int32_t a = body_dense(sp.rigid_body_a), b = body_dense(sp.rigid_body_b);
+        if (a < 0 || b < 0 || a == b) {
+            structure_error_ = true;
+            structure_msg_ = "spring endpoints must be two existing, distinct bodies (springs.rs:26-29 unwrap)";
+            a = b = 0;
+        }
+        sp_a.push_back(a);
+        sp_b.push_back(b);
+        sp_rest.push_back(sp.rest_length);
+        sp_k.push_back(sp.stiffness);
+        sp_c.push_back(sp.damping);
+    }
+    if (!structure_error_ && ns > 0) {
+        for (uint32_t s = 0; s < ns; ++s) {
+            bs_count[sp_a[s]]++;
+            bs_count[sp_b[s]]++;
+        }
+        for (uint32_t i = 0; i < nb; ++i) bs_start[i + 1] = bs_start[i] + bs_count[i];
+        bs_list.resize(bs_start[nb]);
+        std::vector<uint32_t> fillp(bs_start.begin(), bs_start.end() - 1);
+        for (uint32_t s = 0; s < ns; ++s) {
+            bs_list[fillp[sp_a[s]]++] = s;
+            bs_list[fillp[sp_b[s]]++] = s;
+        }
+    }
+
+    // joints (arena order), grouped into components of the body graph
+    const uint32_t nj = joints_.len();
+    std::vector<int32_t> j_a, j_b;
+    std::vector<float2> j_an_a, j_an_b;
+    std::vector<float> j_dist, j_target;
+    for (uint32_t s = 0; s < joints_.capacity_slots(); ++s) {
+        if (!joints_.occupied(s)) continue;
+        const bg_fixed_joint& j = joints_.at_slot(s);
+        int32_t a = body_dense(j.rigid_body_a), b = body_dense(j.rigid_body_b);
+        if (a < 0 || b < 0 || a == b) {
+            structure_error_ = true;
+            structure_msg_ = "joint bodies must exist and differ (physics.rs:427-432 unwrap)";
+            a = b = 0;
+        }
+        j_a.push_back(a);
+        j_b.push_back(b);
+        j_an_a.push_back(make_float2(j.anchor_a[0], j.anchor_a[1]));
+        j_an_b.push_back(make_float2(j.anchor_b[0], j.anchor_b[1]));
+        j_dist.push_back(j.distance);
+        j_target.push_back(j.target_angle);
+    }
+    std::vector<uint32_t> jc_start, jc_list;
+    uint32_t n_jcomp = 0;
+    if (nj > 0 && !structure_error_) {
+        std::vector<int32_t> uf(nb);
+        std::iota(uf.begin(), uf.end(), 0);
+        auto find = [&](int32_t x) {
+            while (uf[x] != x) {
+                uf[x] = uf[uf[x]];
+                x = uf[x];
+            }
+            return x;
+        };
+        for (uint32_t k = 0; k < nj; ++k) {
+            int32_t ra = find(j_a[k]), rb = find(j_b[k]);
+            if (ra != rb) uf[ra] = rb;
+        }
+        std::vector<int32_t> comp_of_root(nb, -1);
+        std::vector<uint32_t> jcomp(nj), counts;
+        for (uint32_t k = 0; k < nj; ++k) {
+            int32_t r = find(j_a[k]);
+            if (comp_of_root[r] < 0) {
+                comp_of_root[r] = (int32_t)counts.size();
+                counts.push_back(0);
+            }
+            jcomp[k] = (uint32_t)comp_of_root[r];
+            counts[jcomp[k]]++;
+        }
+        n_jcomp = (uint32_t)counts.size();
+        jc_start.assign(n_jcomp + 1, 0);
+        for (uint32_t c = 0; c < n_jcomp; ++c) jc_start[c + 1] = jc_start[c] + counts[c];
+        jc_list.resize(nj);
+        std::vector<uint32_t> fillp(jc_start.begin(), jc_start.end() - 1);
+        for (uint32_t k = 0; k < nj; ++k) jc_list[fillp[jcomp[k]]++] = k;
+    }
+
+    // constraints
+    const uint32_t nk = (uint32_t)constraints_.size();
+    std::vector<float2> con_pos(nk);
+    std::vector<float> con_r(nk);
+    for (uint32_t k = 0; k < nk; ++k) {
+        con_pos[k] = make_float2(constraints_[k].x, constraints_[k].y);
+        con_r[k] = constraints_[k].r;
+    }
+
+    // path: one collider per body in the same dense order, no springs/joints
+    bool sphere = (nb == nc) && ns == 0 && nj == 0 && nb > 0;
+    for (uint32_t i = 0; sphere && i < nb; ++i)
+        if (bc_start[i + 1] - bc_start[i] != 1 || bc_list[bc_start[i]] != i || cpar[i] != (int32_t)i) sphere = false;
+    path_ = sphere ? StepPath::Sphere : StepPath::General;
+    if (sphere) {
+        for (uint32_t i = 0; i < nb; ++i)
+            if (cfl[i] & 1u) bflags[i] |= BF_SENSOR;
+    }
+    // common drift for the broadphase query centre: worth it when every
+    // moving body feels the same gravity
+    {
+        uint32_t dyn = 0;
+        bool uniform = true;
+        for (uint32_t i = 0; i < nb; ++i) {
+            if (bflags[i] & BF_STATIC) continue;
+            ++dyn;
+            if (gmod[i] != 1.0f) uniform = false;
+        }
+        use_drift_ = uniform && dyn * 2 >= nb;
+    }
+
+    // allocate
+    DeviceWorld& d = d_;
+    d.nb = nb;
+    d.nc = nc;
+    d.ns = ns;
+    d.nj = nj;
+    d.nk = nk;
+    d.n_jcomp = n_jcomp;
+    d.cells_cap = next_pow2(std::max<uint32_t>(4u * nc, 1024u));
+    d.key_bits = 0;
+    while ((1u << d.key_bits) < d.cells_cap) ++d.key_bits;
+    const bool two = sphere;  // the sphere path permutes into a second copy
+    for (int copy = 0; copy < (two ? 2 : 1); ++copy) {
+        BodyArrays& b = copy ? d.b_alt : d.b;
+        HIP_TRY(dmalloc(&b.pos, nb));
+        HIP_TRY(dmalloc(&b.pos_old, nb));
+        HIP_TRY(dmalloc(&b.vel, nb));
+        HIP_TRY(dmalloc(&b.acc, nb));
+        HIP_TRY(dmalloc(&b.vreq, nb));
+        HIP_TRY(dmalloc(&b.rot, nb));
+        HIP_TRY(dmalloc(&b.omega, nb));
+        HIP_TRY(dmalloc(&b.torque, nb));
+        HIP_TRY(dmalloc(&b.inertia, nb));
+        HIP_TRY(dmalloc(&b.mass, nb));
+        HIP_TRY(dmalloc(&b.gmod, nb));
+        HIP_TRY(dmalloc(&b.bflags, nb));
+        HIP_TRY(dmalloc(&b.aid, nb));
+        ColliderArrays& c = copy ? d.c_alt : d.c;
+        HIP_TRY(dmalloc(&c.abs, nc));
+        HIP_TRY(dmalloc(&c.off, nc));
+        HIP_TRY(dmalloc(&c.radius, nc));
+        HIP_TRY(dmalloc(&c.parent, nc));
+        HIP_TRY(dmalloc(&c.groups, nc));
+        HIP_TRY(dmalloc(&c.cflags, nc));
+    }
+    HIP_TRY(dmalloc(&d.abs_next, nc));
+    HIP_TRY(dmalloc(&d.x0, nc));
+    HIP_TRY(dmalloc(&d.bc_start, nb + 1));
+    HIP_TRY(dmalloc(&d.bc_list, bc_list.size()));
+    HIP_TRY(dmalloc(&d.sp_a, ns));
+    HIP_TRY(dmalloc(&d.sp_b, ns));
+    HIP_TRY(dmalloc(&d.sp_rest, ns));
+    HIP_TRY(dmalloc(&d.sp_k, ns));
+    HIP_TRY(dmalloc(&d.sp_c, ns));
+    HIP_TRY(dmalloc(&d.bs_start, nb + 1));
+    HIP_TRY(dmalloc(&d.bs_list, bs_list.size()));
+    HIP_TRY(dmalloc(&d.j_a, nj));
+    HIP_TRY(dmalloc(&d.j_b, nj));
+    HIP_TRY(dmalloc(&d.j_anchor_a, nj));
+    HIP_TRY(dmalloc(&d.j_anchor_b, nj));
+    HIP_TRY(dmalloc(&d.j_dist, nj));
+    HIP_TRY(dmalloc(&d.j_target, nj));
+    HIP_TRY(dmalloc(&d.jc_start, n_jcomp + 1));
+    HIP_TRY(dmalloc(&d.jc_list, nj));
+    HIP_TRY(dmalloc(&d.con_pos, nk));
+    HIP_TRY(dmalloc(&d.con_r, nk));
+    HIP_TRY(dmalloc(&d.keys, nc));
+    HIP_TRY(dmalloc(&d.vals, nc));
+    HIP_TRY(dmalloc(&d.keys_tmp, nc));
+    HIP_TRY(dmalloc(&d.vals_tmp, nc));
+    HIP_TRY(dmalloc(&d.cell_start, (size_t)d.cells_cap + 1));
+    HIP_TRY(dmalloc(&d.slot_of, nb));
+    HIP_TRY(dmalloc(&d.grid, 1));
+    HIP_TRY(dmalloc(&d.accum, 1));
+    HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, 1)));
+    HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), d.stream));
+    d.allocated = true;
+
+    // copy
+    hipStream_t s = d.stream;
+    HIP_TRY(h2d(d.b.pos, pos, s));
+    HIP_TRY(h2d(d.b.pos_old, pos_old, s));
+    HIP_TRY(h2d(d.b.vel, vel, s));
+    HIP_TRY(h2d(d.b.acc, acc, s));
+    HIP_TRY(h2d(d.b.vreq, vreq, s));
+    HIP_TRY(h2d(d.b.rot, rot, s));
+    HIP_TRY(h2d(d.b.omega, omega, s));
+    HIP_TRY(h2d(d.b.torque, torque, s));
+    HIP_TRY(h2d(d.b.inertia, inertia, s));
+    HIP_TRY(h2d(d.b.mass, mass, s));
+    HIP_TRY(h2d(d.b.gmod, gmod, s));
+    HIP_TRY(h2d(d.b.bflags, bflags, s));
+    HIP_TRY(h2d(d.b.aid, aid, s));
+    HIP_TRY(h2d(d.c.abs, cabs, s));
+    HIP_TRY(h2d(d.c.off, coff, s));
+    HIP_TRY(h2d(d.c.radius, crad, s));
+    HIP_TRY(h2d(d.c.parent, cpar, s));
+    HIP_TRY(h2d(d.c.groups, cgrp, s));
+    HIP_TRY(h2d(d.c.cflags, cfl, s));
+    HIP_TRY(h2d(d.bc_start, bc_start, s));
+    HIP_TRY(h2d(d.bc_list, bc_list, s));
+    HIP_TRY(h2d(d.sp_a, sp_a, s));
+    HIP_TRY(h2d(d.sp_b, sp_b, s));
+    HIP_TRY(h2d(d.sp_rest, sp_rest, s));
+    HIP_TRY(h2d(d.sp_k, sp_k, s));
+    HIP_TRY(h2d(d.sp_c, sp_c, s));
+    HIP_TRY(h2d(d.bs_start, bs_start, s));
+    HIP_TRY(h2d(d.bs_list, bs_list, s));
+    HIP_TRY(h2d(d.j_a, j_a, s));
+    HIP_TRY(h2d(d.j_b, j_b, s));
+    HIP_TRY(h2d(d.j_anchor_a, j_an_a, s));
+    HIP_TRY(h2d(d.j_anchor_b, j_an_b, s));
+    HIP_TRY(h2d(d.j_dist, j_dist, s));
+    HIP_TRY(h2d(d.j_target, j_target, s));
+    HIP_TRY(h2d(d.jc_start, jc_start, s));
+    HIP_TRY(h2d(d.jc_list, jc_list, s));
+    HIP_TRY(h2d(d.con_pos, con_pos, s));
+    HIP_TRY(h2d(d.con_r, con_r, s));
+    StepAccum acc0;
+    memset(&acc0, 0, sizeof(acc0));
+    bbox_parity_ = 0;
+    acc0.bbox[0][0] = f2ord_host(minx);
+    acc0.bbox[0][1] = f2ord_host(miny);
+    acc0.bbox[0][2] = f2ord_host(maxx);
+    acc0.bbox[0][3] = f2ord_host(maxy);
+    acc0.bbox[1][0] = acc0.bbox[1][1] = 0xFFFFFFFFu;
+    acc0.old_dt = params_.old_dt;
+    HIP_TRY(hipMemcpyAsync(d.accum, &acc0, sizeof(acc0), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // host vectors die at return
+
+    dev_dirty_ = false;
+    host_stale_ = false;
+    stepped_since_upload_ = false;
+    fresh_ = true;
+    need_rebuild_ = true;
+    steps_since_rebuild_ = 0;
+    return BG_OK;
+}
+
+// Device -> host mirror, in arena order.
+int World::download() {
+    DeviceWorld& d = d_;
+    if (!d.allocated) {
+        host_stale_ = false;
+        return BG_OK;
+    }
+    hipStream_t s = d.stream;
+    const uint32_t nb = d.nb, nc = d.nc;
+    std::vector<float2> pos, pos_old, vel, cabs;
+    std::vector<float> rot, omega;
+    std::vector<uint32_t> bflags, aid;
+    HIP_TRY(d2h(pos, d.b.pos, nb, s));
+    HIP_TRY(d2h(pos_old, d.b.pos_old, nb, s));
+    HIP_TRY(d2h(vel, d.b.vel, nb, s));
+    HIP_TRY(d2h(rot, d.b.rot, nb, s));
+    HIP_TRY(d2h(omega, d.b.omega, nb, s));
+    HIP_TRY(d2h(bflags, d.b.bflags, nb, s));
+    HIP_TRY(d2h(aid, d.b.aid, nb, s));
+    HIP_TRY(d2h(cabs, d.c.abs, nc, s));
+    StepAccum acc;
+    HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    params_.old_dt = acc.old_dt;
+    // in the sphere path slot k holds dense body aid[k] (and its collider)
+    for (uint32_t k = 0; k < nb; ++k) {
+        uint32_t dense = (path_ == StepPath::Sphere) ? aid[k] : k;
+        bg_rigid_body& f = bodies_.at_slot(body_slot_of_dense_[dense]).f;
+        f.position[0] = pos[k].x;
+        f.position[1] = pos[k].y;
+        f.position_old[0] = pos_old[k].x;
+        f.position_old[1] = pos_old[k].y;
+        f.calculated_velocity[0] = vel[k].x;
+        f.calculated_velocity[1] = vel[k].y;
+        f.rotation = rot[k];
+        f.angular_velocity = omega[k];
+        f.has_velocity_request = (bflags[k] & BF_HAS_VREQ) ? 1u : 0u;
+        if (stepped_since_upload_) {
+            f.acceleration[0] = f.acceleration[1] = 0.0f;  // physics.rs:329,354
+            if (f.body_type != BG_STATIC) f.torque = 0.0f;  // physics.rs:350,355
+        }
+    }
+    for (uint32_t k = 0; k < nc; ++k) {
+        uint32_t dense = (path_ == StepPath::Sphere) ? aid[k] : k;
+        bg_collider& f = colliders_.at_slot(col_slot_of_dense_[dense]).f;
+        f.absolute_translation[0] = cabs[k].x;
+        f.absolute_translation[1] = cabs[k].y;
+        // matrix part of body.transform() * offset (physics.rs:363); unused by the step
+        if (f.has_parent) {
+            const HostBody* hb = bodies_.get(f.parent);
+            if (hb) {
+                float sn = sinf(hb->f.rotation), cs = cosf(hb->f.rotation);
+                for (int col = 0; col < 2; ++col) {
+                    float x = f.offset_matrix[2 * col], y = f.offset_matrix[2 * col + 1];
+                    f.absolute_matrix[2 * col] = cs * x + (-sn) * y;
+                    f.absolute_matrix[2 * col + 1] = sn * x + cs * y;
+                }
+            }
+        }
+    }
+    host_stale_ = false;
+    return BG_OK;
+}
+
+int World::rebuild() {
+    DeviceWorld& d = d_;
+    float cell = (rmax_ > 0.0f) ? cell_scale_ * 2.0f * rmax_ : 1.0f;
+    if (!isfinite(cell)) cell = 1.0f;
+    const bool sphere = path_ == StepPath::Sphere;
+    if (int st = stage_begin(BG_STAGE_REBUILD)) return st;
+    HIP_TRY(launch_rebuild_keys(d, cell, bbox_parity_, sphere));
+    HIP_TRY(launch_rebuild_sort_and_cells(d));
+    if (sphere) HIP_TRY(launch_sphere_permute(d, fresh_));
+    if (int st = stage_end(BG_STAGE_REBUILD)) return st;
+    bbox_parity_ ^= 1u;
+    drift_x_ = drift_y_ = vdrift_x_ = vdrift_y_ = 0.0;
+    h_prev_ = 0.0;
+    need_rebuild_ = false;
+    steps_since_rebuild_ = 0;
+    return BG_OK;
+}
+
+int World::run_substep(float h, bool fresh) {
+    DeviceWorld& d = d_;
+    SubstepArgs a;
+    a.h = h;
+    a.ratio_rest = h / h;
+    a.gx = params_.gravity[0];
+    a.gy = params_.gravity[1];
+    a.rmax = rmax_;
+    a.drift_prev_x = (float)drift_x_;
+    a.drift_prev_y = (float)drift_y_;
+    if (use_drift_) {
+        double hh = (double)h;
+        double scale = h_prev_ > 0.0 ? hh / h_prev_ : 0.0;
+        vdrift_x_ = vdrift_x_ * scale + (double)params_.gravity[0] * hh * hh;
+        vdrift_y_ = vdrift_y_ * scale + (double)params_.gravity[1] * hh * hh;
+        drift_x_ += vdrift_x_;
+        drift_y_ += vdrift_y_;
+        h_prev_ = hh;
+    }
+    a.drift_x = (float)drift_x_;
+    a.drift_y = (float)drift_y_;
+    a.first_dyn = first_dyn_;
+    a.fresh = fresh ? 1u : 0u;
+    a.collisions = params_.collisions_enabled ? 1u : 0u;
+    a.parity = bbox_parity_;
+    a.has_springs = d.ns > 0 ? 1u : 0u;
+    a.inertia_all_pos = 0;
+    if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
+    if (path_ == StepPath::Sphere) {
+        HIP_TRY(launch_sphere_substep(d, a, groups_, sensors_, zero_off_, d.c.abs, d.abs_next));
+        float2* t = d.c.abs;
+        d.c.abs = d.abs_next;
+        d.abs_next = t;
+    } else {
+        HIP_TRY(launch_general_substep(d, a, params_.joint_iterations, groups_, sensors_));
+    }
+    return stage_end(BG_STAGE_SUBSTEP);
+}
+
+int World::check_errors() {
+    DeviceWorld& d = d_;
+    StepAccum acc;
+    HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    uint32_t panic_bits = acc.error_flags & (ERR_NAN_NORMAL | ERR_JOINT_MASS | ERR_JOINT_ROT);
+    if (acc.error_flags) {
+        uint32_t zero = 0;
+        HIP_TRY(hipMemcpyAsync(&d.accum->error_flags, &zero, 4, hipMemcpyHostToDevice, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+    }
+    if (panic_bits & ERR_NAN_NORMAL) return fail(BG_ERR_PANIC, "assertion failed: !n.is_nan() (physics.rs:293)");
+    if (panic_bits & ERR_JOINT_MASS)
+        return fail(BG_ERR_PANIC, "assertion failed: body calculated_mass > 0.0 (physics.rs:447-448)");
+    if (panic_bits & ERR_JOINT_ROT)
+        return fail(BG_ERR_PANIC, "assertion failed: body rotation is finite (physics.rs:471-474)");
+    return BG_OK;
+}
+
+int World::step(double delta, uint32_t n) {
+    if (n == 0) return BG_OK;
+    int st = ensure_device();
+    if (st) return st;
+    if (dev_dirty_) {
+        st = upload();
+        if (st) return st;
+    }
+    if (params_.collisions_enabled && params_.use_spatial_hash)
+        return fail(BG_ERR_UNSUPPORTED, "spatial collisions not supported right now (physics.rs:411-412)");
+    if (structure_error_) return fail(BG_ERR_PANIC, structure_msg_);
+    for (uint32_t k = 0; k < n; ++k) {
+        const float df = (float)delta;                             // physics.rs:80
+        const float h = df / (float)params_.substeps;              // physics.rs:399
+        for (uint32_t sub = 0; sub < params_.substeps; ++sub) {
+            if (need_rebuild_ && params_.collisions_enabled) {
+                st = rebuild();
+                if (st) return st;
+            }
+            st = run_substep(h, fresh_);
+            if (st) return st;
+            fresh_ = false;
+            stepped_since_upload_ = true;
+        }
+        params_.time += delta;  // physics.rs:81
+        if (++steps_since_rebuild_ >= rebuild_interval_) need_rebuild_ = true;
+    }
+    host_stale_ = true;
+    return check_errors();
+}
+
+// Physics::fixed_step, physics.rs:84-99.
+int World::fixed_step(double frame_time) {
+    params_.accumulator += frame_time;
+    const double delta = 1.0 / 60.0;
+    int max_steps = 3;
+    while (params_.accumulator >= delta && max_steps > 0) {
+        int st = step(delta, 1);  // integrate(substeps, delta as f32); time += delta
+        if (st) return st;
+        params_.accumulator -= delta;
+        max_steps -= 1;
+    }
+    return BG_OK;
+}
+
+int World::synchronize() {
+    if (d_.stream) HIP_TRY(hipStreamSynchronize(d_.stream));
+    return BG_OK;
+}
+
+}  // namespace bg

@@ -1,0 +1,140 @@
+// world.hpp -- host side of one `Physics` world: the arenas the reference
+// keeps (physics.rs:11-16), mass setup, and the orchestration of the device
+// step.  The host mirror is authoritative between mutations; after a step
+// the device is, and getters download lazily.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/blobs_gpu.h"
+#include "arena.hpp"
+#include "device_state.hpp"
+
+namespace bg {
+
+struct HostBody {
+    bg_rigid_body f{};
+    std::vector<bg_handle> colliders;  // rbd.colliders, each handle twice (Q2)
+    std::vector<bg_handle> joints;     // rbd.connected_joints
+};
+
+struct HostCollider {
+    bg_collider f{};
+};
+
+struct HostConstraint {
+    float x, y, r;
+};
+
+enum class StepPath : uint32_t { None = 0, General = 1, Sphere = 2 };
+
+struct Status {
+    int code = BG_OK;
+    std::string msg;
+};
+
+class World {
+public:
+    World(float gx, float gy, bool use_spatial_hash);
+    ~World();
+
+    // construction (physics.rs:121-239)
+    int insert_rbd(const bg_rigid_body& b, bg_handle* out);
+    int insert_collider_with_parent(const bg_collider& c, bg_handle parent, bg_handle* out);
+    int insert_rbds(uint32_t n, const bg_body_batch& b, bg_handle* out);
+    int insert_colliders(uint32_t n, const bg_collider_batch& c, const bg_handle* parents, bg_handle* out);
+    int create_fixed_joint(bg_handle a, bg_handle b, const float* aa, const float* ab, bool with_distance,
+                           float distance, bg_handle* out);
+    int insert_spring(const bg_spring& s, bg_handle* out);
+    int push_constraint(float x, float y, float r);
+    int remove_rbd(bg_handle h);
+    int remove_col(bg_handle h);
+    int reset();
+
+    // stepping
+    int step(double delta, uint32_t n);
+    int fixed_step(double frame_time);
+    int synchronize();
+
+    // access
+    int get_params(bg_params* p);
+    int set_params(const bg_params* p);
+    int get_rbd(bg_handle h, bg_rigid_body* out);
+    int set_rbd(bg_handle h, const bg_rigid_body* in);
+    int get_col(bg_handle h, bg_collider* out);
+    int get_bodies(float* pos, float* pos_old, float* vel, float* rot, float* omega, bg_handle* handles);
+    int get_collider_positions(float* abs, uint32_t* count);
+    int contact_count(uint64_t* out);
+    int set_broadphase(float cell_scale, uint32_t rebuild_interval);
+    int set_profiling(bool enable);
+    int stage_stats(uint32_t stage, uint64_t* launches, double* total_ms);
+    uint32_t rbd_count() const { return bodies_.len(); }
+    StepPath path() const { return path_; }
+
+    int fail(int code, const std::string& msg) {
+        last_error = msg;
+        return code;
+    }
+    std::string last_error;
+
+private:
+    void update_mass_and_inertia(HostBody& b);
+    int ensure_device();
+    int upload();
+    int download();
+    int sync_host() { return host_stale_ ? download() : BG_OK; }
+    void mark_dirty() { dev_dirty_ = true; }
+    int rebuild();
+    int run_substep(float h, bool fresh);
+    int check_errors();
+    void free_device();
+    void free_arrays();
+    int stage_begin(uint32_t stage);
+    int stage_end(uint32_t stage);
+    int collect_stage_times();
+
+    Arena<HostBody> bodies_;
+    Arena<HostCollider> colliders_;
+    Arena<bg_fixed_joint> joints_;
+    Arena<bg_spring> springs_;
+    std::vector<HostConstraint> constraints_;
+    bg_params params_{};
+
+    // device mirror
+    DeviceWorld d_{};
+    StepPath path_ = StepPath::None;
+    bool dev_dirty_ = true;
+    bool host_stale_ = false;
+    bool stepped_since_upload_ = false;
+    bool fresh_ = true;
+    bool need_rebuild_ = true;
+    uint32_t steps_since_rebuild_ = 0;
+    uint32_t bbox_parity_ = 0;
+    float cell_scale_ = 1.0f;
+    uint32_t rebuild_interval_ = 4;
+    // world properties fixed at upload
+    float rmax_ = 0.0f;
+    int32_t first_dyn_ = -1;
+    bool groups_ = false, sensors_ = false, zero_off_ = false, use_drift_ = false;
+    bool structure_error_ = false;
+    std::string structure_msg_;
+    // dense <-> slot maps of the last upload
+    std::vector<uint32_t> body_slot_of_dense_, col_slot_of_dense_;
+    std::vector<int32_t> body_dense_of_slot_, col_dense_of_slot_;
+    // broadphase drift of the current rebuild period
+    double drift_x_ = 0, drift_y_ = 0, vdrift_x_ = 0, vdrift_y_ = 0, h_prev_ = 0;
+    uint64_t contact_base_ = 0;
+    // profiling: event pairs recorded since the last collection, per stage
+    bool profiling_ = false;
+    std::vector<hipEvent_t> event_pool_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_[BG_STAGE_COUNT];
+    hipEvent_t open_[BG_STAGE_COUNT] = {nullptr, nullptr};
+    uint64_t stage_launches_[BG_STAGE_COUNT] = {0, 0};
+    double stage_ms_[BG_STAGE_COUNT] = {0.0, 0.0};
+};
+
+}  // namespace bg

@@ -1,0 +1,241 @@
+/*
+ * blobs_gpu.h -- C ABI of the MI355X-native `Physics::step` drop-in.
+ *
+ * The reference (darthdeus/blobs) is a Rust crate with no FFI; its boundary is
+ * the `Physics` value and its methods (blobs/src/physics.rs:3-239).  Every
+ * entry point below replaces one of those methods (cited per function) so a
+ * Rust `-sys` crate can bind them 1:1 and keep the reference API on top.
+ *
+ * Conventions
+ *  - Every call returns a bg_status (0 = OK).  No exception or panic crosses
+ *    the ABI.  Where the reference panics or asserts (unwrap on a missing
+ *    body, assert! in the solver), the call returns BG_ERR_PANIC and
+ *    bg_last_error() holds the message; a Rust facade maps that to panic!.
+ *  - Handles are thunderdome `Index::to_bits()` values:
+ *    (generation << 32) | slot.  Iteration order is slot order, as in the
+ *    reference arenas.
+ *  - A world owns its device memory and one HIP stream.  Calls on one world
+ *    are not thread-safe (the reference `Physics` is !Send); distinct worlds
+ *    may be driven from distinct threads.
+ *  - Host-side construction (insert, create joint, mass setup) needs no GPU.
+ *    bg_step and friends need one and return BG_ERR_NO_DEVICE otherwise;
+ *    there is no CPU fallback.
+ */
+#ifndef BLOBS_GPU_H
+#define BLOBS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BG_ABI_VERSION 1
+
+typedef struct bg_world bg_world;
+typedef uint64_t bg_handle;
+
+typedef enum {
+    BG_OK = 0,
+    BG_ERR_INVALID_ARGUMENT = 1,
+    BG_ERR_NO_DEVICE = 2,   /* no HIP device: GPU entry points refuse to run */
+    BG_ERR_HIP = 3,         /* a HIP runtime call failed */
+    BG_ERR_NOT_FOUND = 4,   /* stale or unknown handle (reference: None) */
+    BG_ERR_PANIC = 5,       /* the reference would panic/assert here */
+    BG_ERR_CAPACITY = 6,    /* a fixed-size device buffer overflowed */
+    BG_ERR_UNSUPPORTED = 7, /* reference path that itself panics */
+} bg_status;
+
+/* RigidBodyType, rigid_body.rs:221-242 (same discriminants). */
+enum { BG_DYNAMIC = 0, BG_STATIC = 1, BG_KINEMATIC_POSITION_BASED = 2, BG_KINEMATIC_VELOCITY_BASED = 3 };
+
+/* RigidBody fields, rigid_body.rs:42-74.  Used by insert (as the builder
+ * output, RigidBodyBuilder::build rigid_body.rs:376-400) and by get/set. */
+typedef struct {
+    float position[2];
+    float position_old[2];
+    float center_of_mass[2];
+    float calculated_mass;
+    float gravity_mod;
+    float rotation;
+    float angular_velocity;
+    float torque;
+    float inertia;
+    float scale[2];
+    float acceleration[2];
+    float velocity_request[2];
+    uint32_t has_velocity_request; /* Option<Vec2> */
+    float calculated_velocity[2];
+    uint32_t body_type;
+    uint64_t user_data[2]; /* u128, little-endian halves */
+} bg_rigid_body;
+
+/* Collider fields, collider.rs:4-20 (shape omitted: physics reads radius,
+ * collider.rs:11).  Affine2 = Mat2 columns (x_axis, y_axis) + translation. */
+typedef struct {
+    float offset_matrix[4];
+    float offset_translation[2];
+    float absolute_matrix[4];
+    float absolute_translation[2];
+    uint64_t user_data[2];
+    bg_handle parent; /* 0 with has_parent = 0 means None */
+    uint32_t has_parent;
+    float radius;
+    float mass_override;
+    uint32_t has_mass_override;
+    uint32_t is_sensor;
+    uint32_t memberships; /* InteractionGroups, groups.rs:7-12 */
+    uint32_t filter;
+} bg_collider;
+
+/* Spring, springs.rs:16-22. */
+typedef struct {
+    bg_handle rigid_body_a;
+    bg_handle rigid_body_b;
+    float rest_length;
+    float stiffness;
+    float damping;
+} bg_spring;
+
+/* FixedJoint, joints.rs:7-19. */
+typedef struct {
+    bg_handle rigid_body_a;
+    bg_handle rigid_body_b;
+    float anchor_a[2];
+    float anchor_b[2];
+    float distance;
+    float target_angle;
+} bg_fixed_joint;
+
+/* Physics pub fields that steer the step, physics.rs:6-9,20,27,30-33. */
+typedef struct {
+    float gravity[2];
+    uint32_t substeps;         /* default 8, physics.rs:46 */
+    uint32_t joint_iterations; /* default 4, physics.rs:47 */
+    uint32_t collisions_enabled;
+    uint32_t use_spatial_hash; /* true makes the reference panic, physics.rs:411-412 */
+    double accumulator;
+    double time;
+    float old_dt;
+} bg_params;
+
+/* Bulk body/collider description, structure-of-arrays.  Inserting n entries
+ * with one call is defined as n sequential single inserts. */
+typedef struct {
+    const float *position;         /* [2n] */
+    const float *velocity_request; /* [2n] or NULL */
+    const uint8_t *has_velocity_request; /* [n] or NULL (all true if velocity_request given) */
+    const float *gravity_mod;      /* [n] or NULL (1.0) */
+    const float *rotation;         /* [n] or NULL (0.0) */
+    const uint8_t *body_type;      /* [n] or NULL (Dynamic) */
+} bg_body_batch;
+
+typedef struct {
+    const float *offset_translation;   /* [2n] or NULL (identity offset) */
+    const float *absolute_translation; /* [2n] or NULL (identity, quirk of collider.rs:214-215) */
+    const float *radius;               /* [n] */
+    const float *mass_override;        /* [n] or NULL */
+    const uint8_t *has_mass_override;  /* [n] or NULL */
+    const uint8_t *is_sensor;          /* [n] or NULL */
+    const uint32_t *memberships;       /* [n] or NULL (ALL) */
+    const uint32_t *filter;            /* [n] or NULL (ALL) */
+} bg_collider_batch;
+
+/* ---- world lifetime ------------------------------------------------- */
+
+/* Physics::new(gravity, use_spatial_hash), physics.rs:37-69. */
+int bg_world_create(float gravity_x, float gravity_y, int use_spatial_hash, bg_world **out);
+/* Drop for Physics. */
+int bg_world_destroy(bg_world *w);
+/* Physics::reset, physics.rs:71-76 (clears bodies, colliders, joints, springs). */
+int bg_world_reset(bg_world *w);
+
+/* Read/write the pub fields listed in bg_params (physics.rs:3-34). */
+int bg_get_params(bg_world *w, bg_params *out);
+int bg_set_params(bg_world *w, const bg_params *p);
+
+/* ---- construction (host side, no GPU needed) ------------------------ */
+
+/* Physics::insert_rbd, physics.rs:121-128. */
+int bg_insert_rbd(bg_world *w, const bg_rigid_body *body, bg_handle *out);
+/* Physics::insert_collider_with_parent, physics.rs:130-149 (the collider is
+ * pushed twice into rbd.colliders and mass/inertia recomputed, exactly as
+ * collider.rs:179-181 + physics.rs:144-146 do). */
+int bg_insert_collider_with_parent(bg_world *w, const bg_collider *col, bg_handle parent, bg_handle *out);
+/* n x (RigidBodyBuilder::new().position(p).velocity_request(v)...build() +
+ * insert_rbd), the demo recipe of demo/src/simulation.rs:87-97. */
+int bg_insert_rbds(bg_world *w, uint32_t n, const bg_body_batch *b, bg_handle *out);
+/* n x insert_collider_with_parent (physics.rs:130-149). */
+int bg_insert_colliders_with_parents(bg_world *w, uint32_t n, const bg_collider_batch *c, const bg_handle *parents,
+                                     bg_handle *out);
+/* Physics::create_fixed_joint, physics.rs:184-207. */
+int bg_create_fixed_joint(bg_world *w, bg_handle a, bg_handle b, const float anchor_a[2], const float anchor_b[2],
+                          bg_handle *out);
+/* Physics::create_fixed_joint_with_distance, physics.rs:209-239. */
+int bg_create_fixed_joint_with_distance(bg_world *w, bg_handle a, bg_handle b, const float anchor_a[2],
+                                        const float anchor_b[2], float distance, bg_handle *out);
+/* physics.springs.insert(Spring { .. }) (pub arena, physics.rs:14). */
+int bg_insert_spring(bg_world *w, const bg_spring *s, bg_handle *out);
+/* physics.constraints.push(Constraint { position, radius }) (physics.rs:15, lib.rs:189-193). */
+int bg_push_constraint(bg_world *w, float x, float y, float radius);
+/* Physics::remove_rbd, physics.rs:163-172. */
+int bg_remove_rbd(bg_world *w, bg_handle h);
+/* Physics::remove_col, physics.rs:159-161 (ColliderSet::remove, collider.rs:134-164). */
+int bg_remove_col(bg_world *w, bg_handle h);
+
+/* ---- stepping (GPU) -------------------------------------------------- */
+
+/* Physics::step(delta), physics.rs:78-82, repeated n_steps times. */
+int bg_step(bg_world *w, double delta, uint32_t n_steps);
+/* Physics::fixed_step(frame_time), physics.rs:84-99. */
+int bg_fixed_step(bg_world *w, double frame_time);
+/* Wait for queued work on the world's stream. */
+int bg_synchronize(bg_world *w);
+
+/* ---- state access ---------------------------------------------------- */
+
+/* Physics::rbd_count, physics.rs:113-115. */
+int bg_rbd_count(bg_world *w, uint32_t *out);
+/* Physics::get_rbd, physics.rs:105-107 (BG_ERR_NOT_FOUND for None). */
+int bg_get_rbd(bg_world *w, bg_handle h, bg_rigid_body *out);
+/* Physics::get_mut_rbd + write-back of every pub field, physics.rs:109-111. */
+int bg_set_rbd(bg_world *w, bg_handle h, const bg_rigid_body *in);
+/* Physics::get_col, physics.rs:117-119. */
+int bg_get_col(bg_world *w, bg_handle h, bg_collider *out);
+/* Bulk read of body state in arena (iteration) order: each pointer may be
+ * NULL; arrays hold rbd_count entries (x,y interleaved for Vec2 fields). */
+int bg_get_bodies(bg_world *w, float *position, float *position_old, float *velocity, float *rotation,
+                  float *angular_velocity, bg_handle *handles);
+/* Bulk read of collider absolute translations in arena order. */
+int bg_get_collider_positions(bg_world *w, float *absolute_translation, uint32_t *count);
+/* perf counter "collisions" (physics.rs:316): contacts counted since creation. */
+int bg_contact_count(bg_world *w, uint64_t *out);
+/* Text of the last error on this world (or the global one if w is NULL). */
+int bg_last_error(bg_world *w, char *buf, size_t len);
+
+/* ---- tuning ---------------------------------------------------------- */
+
+/* Grid cell size as a multiple of 2*max collider radius (>= 1) and the
+ * number of steps between broadphase rebuilds (>= 1).  Results do not
+ * depend on either; only speed does. */
+int bg_set_broadphase(bg_world *w, float cell_scale, uint32_t rebuild_interval);
+/* Which device path the last step ran: 0 none, 1 general, 2 fused spheres. */
+int bg_step_path(bg_world *w, uint32_t *out);
+
+/* ---- profiling ------------------------------------------------------- */
+
+/* Per-stage timing with HIP events on the world's stream, the analogue of
+ * the reference's tracy spans around `substep` and the contact pass
+ * (lib.rs:195-209, physics.rs:402,242).  Stages: BG_STAGE_SUBSTEP (all
+ * kernels of one substep), BG_STAGE_REBUILD (broadphase rebuild). */
+enum { BG_STAGE_SUBSTEP = 0, BG_STAGE_REBUILD = 1, BG_STAGE_COUNT = 2 };
+int bg_set_profiling(bg_world *w, int enable);
+/* Launch count and summed event time (ms) of a stage since profiling was
+ * enabled (or last reset). */
+int bg_stage_stats(bg_world *w, uint32_t stage, uint64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""Parity of the HIP step (through the C ABI) with the oracle.
+
+Bars (from the path's definition):
+  * integration-only steps and the contact pair set: bit-exact;
+  * contact / spring / joint resolution: max |dx| <= 1e-4 * radius after one
+    step.  Contacts are summed per body in the reference's (i, j<i) order,
+    so sphere scenes are expected bit-exact too and are asserted so.
+"""
+import numpy as np
+import pytest
+
+from blobs_amd import scenes
+from blobs_amd.physics import (ColliderBuilder, Constraint, InteractionGroups, Physics, RigidBodyBuilder, Spring,
+                               STATIC)
+from oracle.oracle import OracleWorld
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bitexact(gpu, ora, what):
+    g, o = bits(gpu), bits(ora)
+    bad = np.argwhere(g != o)
+    assert bad.shape[0] == 0, (f"{what}: {bad.shape[0]} of {g.size} differ; first at {bad[:3].tolist()}: "
+                               f"gpu {np.asarray(gpu)[tuple(bad[0])]} oracle {np.asarray(ora)[tuple(bad[0])]}")
+
+
+def gpu_world(scene, **params):
+    p, handles = scene.build_physics()
+    for k, v in params.items():
+        setattr(p, k, v)
+    return p
+
+
+def compare_state(p, o, tag):
+    b = p.bodies()
+    assert_bitexact(b["position"], o.pos, f"{tag} position")
+    assert_bitexact(b["position_old"], o.pos_old, f"{tag} position_old")
+    assert_bitexact(b["velocity"], o.vel, f"{tag} velocity")
+    assert_bitexact(b["rotation"], o.rot, f"{tag} rotation")
+    assert_bitexact(p.collider_positions(), o.abs_t, f"{tag} collider translation")
+
+
+def test_integration_only_bitexact_600_steps():
+    """Gravity + Verlet + velocity requests (Q1 included), 600 x step(1/60)."""
+    sc = scenes.cfg1_plain_small(steps=600)
+    p = gpu_world(sc, collisions_enabled=False)
+    o = OracleWorld(sc, np.float32, collisions_enabled=False)
+    p.step(sc.dt, 600)
+    o.step(sc.dt, 600)
+    compare_state(p, o, "integration-only")
+    assert p.contact_count() == 0
+
+
+@pytest.mark.parametrize("steps", [1, 30])
+def test_plain_small_contacts_bitexact(steps):
+    """1,024 spheres + static walls, contacts on: positions and contact count bit-exact."""
+    sc = scenes.cfg1_plain_small(steps=steps)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    for k in range(steps):
+        p.step(sc.dt)
+        o.step(sc.dt)
+        assert p.contact_count() == o.contact_count, f"step {k}: contact count"
+    compare_state(p, o, f"{steps} steps")
+    assert p.step_path() == "sphere"
+
+
+def test_general_path_equals_sphere_path():
+    """A zero spring forces the general (multi-kernel) path; it adds +0 to
+    accelerations, so both device paths must agree bit for bit."""
+    sc = scenes.cfg1_plain_small(steps=20)
+    p1 = gpu_world(sc)
+    p2, bh = sc.build_physics()
+    p2.springs.insert(Spring(int(bh[0]), int(bh[1]), 0.0, 0.0, 0.0))
+    p1.step(sc.dt, 20)
+    p2.step(sc.dt, 20)
+    assert p1.step_path() == "sphere" and p2.step_path() == "general"
+    assert_bitexact(p1.bodies()["position"], p2.bodies()["position"], "sphere vs general")
+    assert p1.contact_count() == p2.contact_count()
+
+
+@pytest.mark.parametrize("interval,scale", [(1, 1.0), (3, 1.0), (8, 1.5)])
+def test_broadphase_settings_do_not_change_results(interval, scale):
+    sc = scenes.cfg1_plain_small(steps=12)
+    o = OracleWorld(sc, np.float32)
+    o.step(sc.dt, 12)
+    p = gpu_world(sc)
+    p.set_broadphase(scale, interval)
+    p.step(sc.dt, 12)
+    compare_state(p, o, f"interval {interval} scale {scale}")
+
+
+def test_plain_large_one_step_bitexact_1m():
+    """1,048,576 spheres at packing 0.4: one step(1/120) bit-exact vs the
+    oracle (grid-enumerated, same (i, j) visit order as the O(C^2) loop)."""
+    sc = scenes.cfg2_plain_large(steps=1)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32, use_grid=True)
+    p.step(sc.dt)
+    o.step(sc.dt)
+    assert p.contact_count() == o.contact_count
+    compare_state(p, o, "1M one step")
+
+
+def test_plain_large_invariants_100_steps():
+    """Size-independent properties at the full 1M shape over 100 steps:
+    contacts conserve sum(m x) exactly in reals, so the mass-weighted
+    centroid follows free fall; bodies stay finite; no panic."""
+    sc = scenes.cfg2_plain_large(steps=100)
+    p = gpu_world(sc)
+    m = 4.0 * sc.col_r.astype(np.float64)  # Q2: mass = 2 * (2 r)
+    x0 = sc.pos.astype(np.float64)
+    v0 = sc.vreq.astype(np.float64)
+    p.step(sc.dt, 100)
+    x = p.bodies()["position"].astype(np.float64)
+    assert np.isfinite(x).all()
+    h = sc.dt / 8.0
+    k = 800
+    # first body (Q1) gets velocity * h; the others v: x_k = x0 + k v h + g h^2 k(k+1)/2
+    vel = v0.copy()
+    vel[0] *= h
+    g = np.array(sc.gravity)
+    pred = x0 + k * h * vel + g * h * h * k * (k + 1) / 2.0
+    cm = (m[:, None] * x).sum(0) / m.sum()
+    cm_pred = (m[:, None] * pred).sum(0) / m.sum()
+    assert np.abs(cm - cm_pred).max() < 2e-3, (cm, cm_pred)
+
+
+def test_multi_collider_springs_joints_tolerance():
+    """Multi-collider bodies + springs + fixed joints (general path): one step
+    within 1e-4 * radius of the oracle, and 10 steps with the same contact
+    count when positions stay bit-identical."""
+    sc = scenes.cfg3_multi(n=4096, n_springs=2048, n_joints=1024, steps=10)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    p.step(sc.dt)
+    o.step(sc.dt)
+    b = p.bodies()
+    err = np.abs(b["position"].astype(np.float64) - o.pos.astype(np.float64)).max()
+    assert err <= 1e-4 * float(sc.col_r.min()), err
+    assert p.step_path() == "general"
+
+
+def test_groups_sensors_constraints_static():
+    """Collision groups, sensors, circle constraints and static bodies."""
+    rng = np.random.default_rng(5)
+    n = 600
+    pos = rng.uniform(0, 20, (n, 2)).astype(np.float32)
+    vel = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    rad = rng.uniform(0.2, 0.6, n).astype(np.float32)
+    sc = scenes._single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1 / 60, 10, {})
+    sc.col_mem = rng.choice(np.array([1, 2, 3, 0xFFFFFFFF], np.uint32), n)
+    sc.col_filt = rng.choice(np.array([1, 2, 0xFFFFFFFF], np.uint32), n)
+    sc.col_sensor = (rng.uniform(size=n) < 0.1).astype(np.uint8)
+    sc.body_type = (rng.uniform(size=n) < 0.1).astype(np.uint8)
+    sc.con_pos = np.array([[10.0, 10.0]], np.float32)
+    sc.con_r = np.array([9.0], np.float32)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    for _ in range(10):
+        p.step(sc.dt)
+        o.step(sc.dt)
+        assert p.contact_count() == o.contact_count
+    compare_state(p, o, "groups/sensors/constraints/static")
+
+
+def test_reference_api_construction_and_step():
+    """The reference's own construction recipe, one body at a time."""
+    p = Physics((0.0, -30.0), False)
+    p.constraints.push(Constraint((0.0, 0.0), 4.0))
+    hs = []
+    for k in range(50):
+        x, y = (k % 10) * 0.3 - 1.5, (k // 10) * 0.3
+        h = p.insert_rbd(RigidBodyBuilder().position((x, y)).velocity_request((0.1 * k, 0.0)).build())
+        p.insert_collider_with_parent(
+            ColliderBuilder().radius(0.1 + 0.01 * k).absolute_transform_translation((x, y)).build(), h)
+        hs.append(h)
+    for _ in range(5):
+        p.fixed_step(1 / 60)
+    b = p.get_rbd(hs[3])
+    assert np.all(np.isfinite(b.position))
+    assert p.get_rbd(hs[3]).velocity_request is None
+
+
+def test_push_out_is_flagged_and_separates():
+    """Two colliders of different bodies at the same point (physics.rs:272-286)."""
+    p = Physics((0.0, 0.0), False)
+    hs = []
+    for x in (1.0, 1.0):
+        h = p.insert_rbd(RigidBodyBuilder().position((x, 2.0)).build())
+        p.insert_collider_with_parent(ColliderBuilder().radius(0.5).absolute_transform_translation((x, 2.0)).build(),
+                                      h)
+        hs.append(h)
+    p.substeps = 1
+    p.step(1 / 60)
+    a, b = p.get_rbd(hs[0]).position, p.get_rbd(hs[1]).position
+    assert abs(abs(a[0] - b[0]) - 1.0) < 1e-5
+
+
+def test_empty_and_single_body_worlds():
+    p = Physics((0.0, -9.81), False)
+    p.step(1 / 60)  # nothing to do
+    h = p.insert_rbd(RigidBodyBuilder().position((0.0, 0.0)).build())
+    p.step(1 / 60)
+    assert p.get_rbd(h).position[1] < 0.0
