@@ -107,27 +107,49 @@ def test_plain_large_one_step_bitexact_1m():
 
 
 def test_plain_large_invariants_100_steps():
-    """Size-independent properties at the full 1M shape over 100 steps:
-    contacts conserve sum(m x) exactly in reals, so the mass-weighted
-    centroid follows free fall; bodies stay finite; no panic."""
+    """Size-independent properties at the full 1M shape over 100 steps.
+
+    The reference's f32 trajectory is not a free fall of the centroid: for
+    |y| above ~700 the gravity increment g*h^2 ~ 1e-5 is below half an ulp of
+    the position and `p + (disp + a h^2)` drops it (the f32 oracle shows the
+    same), so the checks are exact ones instead: the result does not depend
+    on the broadphase rebuild interval or cell size (the pair set is exact
+    for every setting), positions stay finite, and along x (no gravity) the
+    mass-weighted centroid follows the initial velocities, because contacts
+    conserve sum(m x)."""
     sc = scenes.cfg2_plain_large(steps=100)
-    p = gpu_world(sc)
-    m = 4.0 * sc.col_r.astype(np.float64)  # Q2: mass = 2 * (2 r)
-    x0 = sc.pos.astype(np.float64)
-    v0 = sc.vreq.astype(np.float64)
-    p.step(sc.dt, 100)
-    x = p.bodies()["position"].astype(np.float64)
+    p1 = gpu_world(sc)
+    p2 = gpu_world(sc)
+    p2.set_broadphase(1.5, 1)
+    p1.step(sc.dt, 100)
+    p2.step(sc.dt, 100)
+    x = p1.bodies()["position"]
     assert np.isfinite(x).all()
+    assert_bitexact(x, p2.bodies()["position"], "rebuild every 4 steps vs every step (cell x1.5)")
+    assert p1.contact_count() == p2.contact_count()
+    m = 4.0 * sc.col_r.astype(np.float64)  # Q2: mass = 2 * (2 r)
     h = sc.dt / 8.0
     k = 800
-    # first body (Q1) gets velocity * h; the others v: x_k = x0 + k v h + g h^2 k(k+1)/2
-    vel = v0.copy()
-    vel[0] *= h
-    g = np.array(sc.gravity)
-    pred = x0 + k * h * vel + g * h * h * k * (k + 1) / 2.0
-    cm = (m[:, None] * x).sum(0) / m.sum()
-    cm_pred = (m[:, None] * pred).sum(0) / m.sum()
-    assert np.abs(cm - cm_pred).max() < 2e-3, (cm, cm_pred)
+    vx = sc.vreq[:, 0].astype(np.float64)
+    vx[0] *= h  # Q1: the first body's request is scaled by h / old_dt = h
+    pred = sc.pos[:, 0].astype(np.float64) + k * h * vx
+    cm = (m * x[:, 0]).sum() / m.sum()
+    assert abs(cm - (m * pred).sum() / m.sum()) < 1e-3
+
+
+def test_plain_large_sphere_vs_general_path_20_steps():
+    """At 1M the fused sphere kernel (cell-ordered bodies) and the general
+    multi-kernel path (arena-ordered bodies, sorted collider index) are two
+    independent device implementations of the same step: bit-identical."""
+    sc = scenes.cfg2_plain_large(steps=20)
+    p1 = gpu_world(sc)
+    p2, bh = sc.build_physics()
+    p2.springs.insert(Spring(int(bh[0]), int(bh[1]), 0.0, 0.0, 0.0))
+    p1.step(sc.dt, 20)
+    p2.step(sc.dt, 20)
+    assert p1.step_path() == "sphere" and p2.step_path() == "general"
+    assert_bitexact(p1.bodies()["position"], p2.bodies()["position"], "sphere vs general at 1M")
+    assert p1.contact_count() == p2.contact_count()
 
 
 def test_multi_collider_springs_joints_tolerance():
