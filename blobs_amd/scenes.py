@@ -237,7 +237,9 @@ def cfg3_multi(n=65536, n_springs=32768, n_joints=16384, steps=300, seed=3, phi=
     radius 2, collider r~U[0.2,0.5]), n_springs springs and n_joints fixed
     joints between random bodies within distance 6 (rest = initial distance,
     stiffness logU[10,1e4], damping 0; joints create_fixed_joint(a,b,0,0)),
-    dt=1/60, gravity on.  Box chosen for collider packing fraction phi."""
+    dt=1/60, gravity on.  Square box centred on the origin, side chosen for
+    collider packing fraction phi (centring keeps f32 position resolution
+    fine: joint rotations reach positions through collider offsets)."""
     rng = Rng(seed)
     cap = n * 8
     bpos = np.zeros((n, 2), np.float32)
@@ -248,8 +250,9 @@ def cfg3_multi(n=65536, n_springs=32768, n_joints=16384, steps=300, seed=3, phi=
     er2 = (0.5 ** 3 - 0.2 ** 3) / (3 * 0.3)
     side = math.sqrt(n * 4.5 * math.pi * er2 / phi)
     forced = C.c_int64()
-    nc = _scenes_lib().sg_multi(rng.st, n, 1, 8, 2.0, 0.2, 0.5, 0.0, 0.0, side, side, 1.0, 100, _f(bpos), _f(bvel),
-                                _i(ncol), _f(off), _f(cr), C.byref(forced))
+    half = side / 2.0
+    nc = _scenes_lib().sg_multi(rng.st, n, 1, 8, 2.0, 0.2, 0.5, -half, -half, half, half, 1.0, 100, _f(bpos),
+                                _f(bvel), _i(ncol), _f(off), _f(cr), C.byref(forced))
     off = off[:nc].copy()
     cr = cr[:nc].copy()
     parent = np.repeat(np.arange(n, dtype=np.int32), ncol)
