@@ -218,8 +218,10 @@ def test_reference_api_construction_and_step():
     assert p.get_rbd(hs[3]).velocity_request is None
 
 
-def test_push_out_is_flagged_and_separates():
-    """Two colliders of different bodies at the same point (physics.rs:272-286)."""
+def test_push_out_pair_matches_oracle():
+    """Two colliders of different bodies at the same point (physics.rs:272-286):
+    +-0.01 push-out, then the contact correction from the new axis; an
+    isolated pair has no order dependence, so the result is exact."""
     p = Physics((0.0, 0.0), False)
     hs = []
     for x in (1.0, 1.0):
@@ -229,8 +231,15 @@ def test_push_out_is_flagged_and_separates():
         hs.append(h)
     p.substeps = 1
     p.step(1 / 60)
-    a, b = p.get_rbd(hs[0]).position, p.get_rbd(hs[1]).position
-    assert abs(abs(a[0] - b[0]) - 1.0) < 1e-5
+    pos = np.array([[1, 2], [1, 2]], np.float32)
+    sc = scenes._single_sphere_scene(pos, np.zeros((2, 2), np.float32), np.array([0.5, 0.5], np.float32),
+                                     (0.0, 0.0), 1 / 60, 1, {})
+    sc.has_vreq[:] = 0
+    o = OracleWorld(sc, np.float32, substeps=1)
+    o.step(1 / 60)
+    got = np.array([p.get_rbd(h).position for h in hs], np.float32)
+    assert_bitexact(got, o.pos, "push-out pair")
+    assert p.contact_count() == o.contact_count == 1
 
 
 def test_empty_and_single_body_worlds():
