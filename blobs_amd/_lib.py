@@ -84,6 +84,17 @@ class SpringC(C.Structure):
     ]
 
 
+class FixedJointC(C.Structure):
+    _fields_ = [
+        ("rigid_body_a", handle_t),
+        ("rigid_body_b", handle_t),
+        ("anchor_a", C.c_float * 2),
+        ("anchor_b", C.c_float * 2),
+        ("distance", C.c_float),
+        ("target_angle", C.c_float),
+    ]
+
+
 class ParamsC(C.Structure):
     _fields_ = [
         ("gravity", C.c_float * 2),
@@ -147,6 +158,8 @@ SIGNATURES = {
     "bg_get_rbd": [WORLD, handle_t, C.POINTER(RigidBodyC)],
     "bg_set_rbd": [WORLD, handle_t, C.POINTER(RigidBodyC)],
     "bg_get_col": [WORLD, handle_t, C.POINTER(ColliderC)],
+    "bg_get_joint": [WORLD, handle_t, C.POINTER(FixedJointC)],
+    "bg_get_spring": [WORLD, handle_t, C.POINTER(SpringC)],
     "bg_get_bodies": [WORLD, f32p, f32p, f32p, f32p, f32p, C.POINTER(handle_t)],
     "bg_get_collider_positions": [WORLD, f32p, u32p],
     "bg_contact_count": [WORLD, u64p],

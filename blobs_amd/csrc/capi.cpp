@@ -157,6 +157,16 @@ int bg_get_col(bg_world* w, bg_handle h, bg_collider* out) {
     return guarded(w, [&](bg::World& x) { return x.get_col(h, out); });
 }
 
+int bg_get_joint(bg_world* w, bg_handle h, bg_fixed_joint* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.get_joint(h, out); });
+}
+
+int bg_get_spring(bg_world* w, bg_handle h, bg_spring* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.get_spring(h, out); });
+}
+
 int bg_get_bodies(bg_world* w, float* position, float* position_old, float* velocity, float* rotation,
                   float* angular_velocity, bg_handle* handles) {
     return guarded(w, [&](bg::World& x) {

@@ -63,6 +63,8 @@ struct DeviceWorld {
     uint32_t *slot_of = nullptr;  // sphere path: dense id -> slot
     GridParams *grid = nullptr;
     StepAccum *accum = nullptr;
+    unsigned long long *block_contacts = nullptr;  // contact count per block of the per-body kernels
+    uint32_t n_block_slots = 0;
     SortTemp sort{};
 
     hipStream_t stream = nullptr;

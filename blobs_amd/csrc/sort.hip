@@ -225,6 +225,44 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, Sor
     return hipGetLastError();
 }
 
+// ---- host-buffer test hooks (bgx_: internal, not part of the Physics ABI) --
+
+extern "C" int bgx_radix_sort_u32(const uint32_t* keys, const uint32_t* vals, uint32_t n, int bits, uint32_t* keys_out,
+                                  uint32_t* vals_out) {
+    uint32_t *k = nullptr, *v = nullptr, *kt = nullptr, *vt = nullptr;
+    SortTemp t;
+    size_t bytes = sizeof(uint32_t) * (n ? n : 1);
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&k, bytes)) || (e = hipMalloc(&v, bytes)) || (e = hipMalloc(&kt, bytes)) ||
+        (e = hipMalloc(&vt, bytes)) || (e = sort_temp_reserve(t, n)))
+        goto done;
+    if ((e = hipMemcpy(k, keys, sizeof(uint32_t) * n, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(v, vals, sizeof(uint32_t) * n, hipMemcpyHostToDevice)))
+        goto done;
+    if ((e = radix_sort_pairs(&k, &v, &kt, &vt, n, bits, t, nullptr))) goto done;
+    if ((e = hipMemcpy(keys_out, k, sizeof(uint32_t) * n, hipMemcpyDeviceToHost)) ||
+        (e = hipMemcpy(vals_out, v, sizeof(uint32_t) * n, hipMemcpyDeviceToHost)))
+        goto done;
+done:
+    (void)hipFree(k), (void)hipFree(v), (void)hipFree(kt), (void)hipFree(vt);
+    sort_temp_free(t);
+    return (int)e;
+}
+
+extern "C" int bgx_exclusive_scan_u32(const uint32_t* in, uint32_t n, uint32_t* out) {
+    uint32_t* d = nullptr;
+    SortTemp t;
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&d, sizeof(uint32_t) * (n ? n : 1))) || (e = sort_temp_reserve(t, n))) goto done;
+    if ((e = hipMemcpy(d, in, sizeof(uint32_t) * n, hipMemcpyHostToDevice))) goto done;
+    if ((e = exclusive_scan_u32(d, d, n, t, nullptr))) goto done;
+    e = hipMemcpy(out, d, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+done:
+    (void)hipFree(d);
+    sort_temp_free(t);
+    return (int)e;
+}
+
 hipError_t radix_sort_pairs(uint32_t** keys, uint32_t** vals, uint32_t** keys_tmp, uint32_t** vals_tmp, uint32_t n,
                             int bits, SortTemp& t, hipStream_t s) {
     if (n <= 1 || bits <= 0) return hipSuccess;

@@ -140,7 +140,19 @@ struct LocalAccum {
     }
 };
 
-__device__ void flush_accum(const LocalAccum& l, StepAccum* g, uint32_t parity, bool with_points) {
+// Only blocks that extend the running extremum issue an atomic (a relaxed
+// read first): thousands of blocks hitting one word serialize otherwise.
+__device__ __forceinline__ void atomic_max_if_greater(uint32_t* p, uint32_t v) {
+    if (v > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, v);
+}
+__device__ __forceinline__ void atomic_min_if_less(uint32_t* p, uint32_t v) {
+    if (v < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, v);
+}
+
+// Contacts go to a per-block slot (summed on the host when queried), never
+// to one shared counter.
+__device__ void flush_accum(const LocalAccum& l, StepAccum* g, unsigned long long* block_contacts, uint32_t parity,
+                            bool with_points) {
     __shared__ uint32_t s_red[8][kThreads / BG_WAVE];
     __shared__ unsigned long long s_cnt[kThreads / BG_WAVE];
     const int lane = threadIdx.x & (BG_WAVE - 1), wid = threadIdx.x / BG_WAVE;
@@ -177,15 +189,15 @@ __device__ void flush_accum(const LocalAccum& l, StepAccum* g, uint32_t parity, 
             s_cnt[0] += s_cnt[w];
         }
         if (with_points) {
-            if (s_red[0][0]) atomicMax(&g->dmax, s_red[0][0]);
-            atomicMin(&g->bbox[parity][0], s_red[1][0]);
-            atomicMin(&g->bbox[parity][1], s_red[2][0]);
-            atomicMax(&g->bbox[parity][2], s_red[3][0]);
-            atomicMax(&g->bbox[parity][3], s_red[4][0]);
+            atomic_max_if_greater(&g->dmax, s_red[0][0]);
+            atomic_min_if_less(&g->bbox[parity][0], s_red[1][0]);
+            atomic_min_if_less(&g->bbox[parity][1], s_red[2][0]);
+            atomic_max_if_greater(&g->bbox[parity][2], s_red[3][0]);
+            atomic_max_if_greater(&g->bbox[parity][3], s_red[4][0]);
         }
         if (s_red[5][0]) atomicOr(&g->error_flags, s_red[5][0]);
         if (s_red[6][0]) atomicAdd(&g->push_outs, s_red[6][0]);
-        if (s_cnt[0]) atomicAdd(&g->contacts, s_cnt[0]);
+        if (s_cnt[0]) block_contacts[blockIdx.x] += s_cnt[0];
     }
 }
 
@@ -286,16 +298,24 @@ __global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float2* __restr
     }
 }
 
-// cell_start[c] = first sorted slot with key >= c, for c in [0, ncells].
+// cell_start[c] = first sorted slot with key >= c, for c in [0, ncells]:
+// one thread per cell, lower_bound over the sorted keys (balanced however
+// the bodies are spread over the grid).
 __global__ __launch_bounds__(kThreads) void k_cell_start(const uint32_t* __restrict__ keys, uint32_t n,
                                                          const GridParams* __restrict__ grid,
                                                          uint32_t* __restrict__ cell_start) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > n) return;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ncells = grid->ncells;
-    int64_t prev = (s == 0) ? -1 : (int64_t)keys[s - 1];
-    int64_t cur = (s == n) ? (int64_t)ncells : (int64_t)keys[s];
-    for (int64_t c = prev + 1; c <= cur; ++c) cell_start[c] = s;
+    if (c > ncells) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    cell_start[c] = lo;
 }
 
 // ============================================================ general path
@@ -468,7 +488,7 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
         la.push_outs = pushes;
         la.flags = flags;
     }
-    flush_accum(la, d.accum, 0, false);
+    flush_accum(la, d.accum, d.block_contacts, 0, false);
 }
 
 // Fixed joints (physics.rs:424-477): Gauss-Seidel in arena order inside each
@@ -582,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
         apply_constraints(s.p, d.con_pos, d.con_r, d.nk);
         d.b.pos[b] = s.p;
     }
-    flush_accum(la, d.accum, a.parity, true);
+    flush_accum(la, d.accum, d.block_contacts, a.parity, true);
 }
 
 // ============================================================ sphere path
@@ -746,7 +766,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(DeviceWorld d, Subs
         la.push_outs = pushes;
         la.flags = flags;
     }
-    flush_accum(la, d.accum, a.parity, true);
+    flush_accum(la, d.accum, d.block_contacts, a.parity, true);
 }
 
 // Gather every per-body array into cell order: dst[s] = src[vals[s]].
@@ -797,7 +817,7 @@ hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity, bool
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d) {
     hipError_t e = radix_sort_pairs(&d.keys, &d.vals, &d.keys_tmp, &d.vals_tmp, d.nc, d.key_bits, d.sort, d.stream);
     if (e != hipSuccess) return e;
-    k_cell_start<<<div_up(d.nc + 1, kThreads), kThreads, 0, d.stream>>>(d.keys, d.nc, d.grid, d.cell_start);
+    k_cell_start<<<div_up(d.cells_cap + 1, kThreads), kThreads, 0, d.stream>>>(d.keys, d.nc, d.grid, d.cell_start);
     return hipGetLastError();
 }
 

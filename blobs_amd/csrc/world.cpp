@@ -395,6 +395,20 @@ int World::set_rbd(bg_handle h, const bg_rigid_body* in) {
     return BG_OK;
 }
 
+int World::get_joint(bg_handle h, bg_fixed_joint* out) {
+    const bg_fixed_joint* j = joints_.get(h);
+    if (!j) return fail(BG_ERR_NOT_FOUND, "joint not found");
+    *out = *j;
+    return BG_OK;
+}
+
+int World::get_spring(bg_handle h, bg_spring* out) {
+    const bg_spring* s = springs_.get(h);
+    if (!s) return fail(BG_ERR_NOT_FOUND, "spring not found");
+    *out = *s;
+    return BG_OK;
+}
+
 int World::get_col(bg_handle h, bg_collider* out) {
     if (sync_host()) return BG_ERR_HIP;
     HostCollider* c = colliders_.get(h);
@@ -444,14 +458,22 @@ int World::get_collider_positions(float* abs, uint32_t* count) {
     return BG_OK;
 }
 
+int World::device_contacts(uint64_t* out) {
+    *out = 0;
+    if (!d_.block_contacts || path_ == StepPath::None) return BG_OK;
+    std::vector<unsigned long long> slots;
+    HIP_TRY(d2h(slots, d_.block_contacts, d_.n_block_slots, d_.stream));
+    HIP_TRY(hipStreamSynchronize(d_.stream));
+    uint64_t s = 0;
+    for (unsigned long long v : slots) s += v;
+    *out = s;
+    return BG_OK;
+}
+
 int World::contact_count(uint64_t* out) {
     uint64_t dev = 0;
-    if (d_.accum && path_ != StepPath::None) {
-        StepAccum acc;
-        HIP_TRY(hipMemcpyAsync(&acc, d_.accum, sizeof(acc), hipMemcpyDeviceToHost, d_.stream));
-        HIP_TRY(hipStreamSynchronize(d_.stream));
-        dev = acc.contacts;
-    }
+    int st = device_contacts(&dev);
+    if (st) return st;
     *out = contact_base_ + dev;
     return BG_OK;
 }
@@ -495,7 +517,7 @@ void World::free_arrays() {
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
     dfree(d.jc_start), dfree(d.jc_list), dfree(d.con_pos), dfree(d.con_r);
     dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.slot_of);
-    dfree(d.grid), dfree(d.accum);
+    dfree(d.grid), dfree(d.accum), dfree(d.block_contacts);
     sort_temp_free(d.sort);
     hipStream_t s = d.stream;
     int dev = d.device;
@@ -515,10 +537,11 @@ void World::free_device() {
 int World::upload() {
     if (d_.stream) HIP_TRY(hipStreamSynchronize(d_.stream));
     // keep the contact counter across re-uploads
-    if (d_.accum && path_ != StepPath::None) {
-        StepAccum acc;
-        HIP_TRY(hipMemcpy(&acc, d_.accum, sizeof(acc), hipMemcpyDeviceToHost));
-        contact_base_ += acc.contacts;
+    {
+        uint64_t dev = 0;
+        int st = device_contacts(&dev);
+        if (st) return st;
+        contact_base_ += dev;
     }
     free_arrays();
 
@@ -810,6 +833,9 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.slot_of, nb));
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
+    d.n_block_slots = (std::max<uint32_t>(nb, 1) + 255) / 256;
+    HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
+    HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
     HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, 1)));
     HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), d.stream));
     d.allocated = true;
@@ -1017,14 +1043,14 @@ int World::check_errors() {
 
 int World::step(double delta, uint32_t n) {
     if (n == 0) return BG_OK;
+    if (params_.collisions_enabled && params_.use_spatial_hash)
+        return fail(BG_ERR_UNSUPPORTED, "spatial collisions not supported right now (physics.rs:411-412)");
     int st = ensure_device();
     if (st) return st;
     if (dev_dirty_) {
         st = upload();
         if (st) return st;
     }
-    if (params_.collisions_enabled && params_.use_spatial_hash)
-        return fail(BG_ERR_UNSUPPORTED, "spatial collisions not supported right now (physics.rs:411-412)");
     if (structure_error_) return fail(BG_ERR_PANIC, structure_msg_);
     for (uint32_t k = 0; k < n; ++k) {
         const float df = (float)delta;                             // physics.rs:80

@@ -66,6 +66,8 @@ public:
     int get_rbd(bg_handle h, bg_rigid_body* out);
     int set_rbd(bg_handle h, const bg_rigid_body* in);
     int get_col(bg_handle h, bg_collider* out);
+    int get_joint(bg_handle h, bg_fixed_joint* out);
+    int get_spring(bg_handle h, bg_spring* out);
     int get_bodies(float* pos, float* pos_old, float* vel, float* rot, float* omega, bg_handle* handles);
     int get_collider_positions(float* abs, uint32_t* count);
     int contact_count(uint64_t* out);
@@ -91,6 +93,7 @@ private:
     int rebuild();
     int run_substep(float h, bool fresh);
     int check_errors();
+    int device_contacts(uint64_t* out);
     void free_device();
     void free_arrays();
     int stage_begin(uint32_t stage);

@@ -203,6 +203,10 @@ int bg_get_rbd(bg_world *w, bg_handle h, bg_rigid_body *out);
 int bg_set_rbd(bg_world *w, bg_handle h, const bg_rigid_body *in);
 /* Physics::get_col, physics.rs:117-119. */
 int bg_get_col(bg_world *w, bg_handle h, bg_collider *out);
+/* physics.joints.get(h) (pub arena, physics.rs:13). */
+int bg_get_joint(bg_world *w, bg_handle h, bg_fixed_joint *out);
+/* physics.springs.get(h) (pub arena, physics.rs:14). */
+int bg_get_spring(bg_world *w, bg_handle h, bg_spring *out);
 /* Bulk read of body state in arena (iteration) order: each pointer may be
  * NULL; arrays hold rbd_count entries (x,y interleaved for Vec2 fields). */
 int bg_get_bodies(bg_world *w, float *position, float *position_old, float *velocity, float *rotation,

@@ -48,6 +48,10 @@ int64_t oracle_contact_pairs_f32(oracle_world_f32 *w, int32_t *pairs_out, int64_
     return contacts_f32(w, pairs_out, cap);
 }
 
+int64_t oracle_contact_pairs_f64(oracle_world_f64 *w, int32_t *pairs_out, int64_t cap) {
+    return contacts_f64(w, pairs_out, cap);
+}
+
 /* Rows i in [i0, i1) of the reference O(C^2) contact loop (physics.rs:248-249),
  * for timing a bounded sample of a large scene.  Returns the pair tests run. */
 int64_t oracle_bruteforce_rows_f32(oracle_world_f32 *w, int32_t i0, int32_t i1) {

@@ -107,6 +107,7 @@ void oracle_fixed_step_f64(oracle_world_f64 *w, double frame_time, double *accum
 /* One contact pass only (physics.rs:241-317), for pair-set checks.
  * pairs_out receives (i, j) of every contact in visit order (cap entries). */
 int64_t oracle_contact_pairs_f32(oracle_world_f32 *w, int32_t *pairs_out, int64_t cap);
+int64_t oracle_contact_pairs_f64(oracle_world_f64 *w, int32_t *pairs_out, int64_t cap);
 
 /* Rows [i0, i1) of the O(C^2) contact loop; returns the number of pair tests. */
 int64_t oracle_bruteforce_rows_f32(oracle_world_f32 *w, int32_t i0, int32_t i1);

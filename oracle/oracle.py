@@ -42,6 +42,8 @@ def lib():
                                                                    C.POINTER(C.c_double)]
         _lib.oracle_contact_pairs_f32.argtypes = [C.POINTER(World32), C.POINTER(C.c_int32), C.c_int64]
         _lib.oracle_contact_pairs_f32.restype = C.c_int64
+        _lib.oracle_contact_pairs_f64.argtypes = [C.POINTER(World64), C.POINTER(C.c_int32), C.c_int64]
+        _lib.oracle_contact_pairs_f64.restype = C.c_int64
         _lib.oracle_bruteforce_rows_f32.argtypes = [C.POINTER(World32), C.c_int32, C.c_int32]
         _lib.oracle_bruteforce_rows_f32.restype = C.c_int64
         f = C.POINTER(C.c_float)
@@ -220,7 +222,8 @@ class OracleWorld:
     def contact_pairs(self, cap=1 << 22):
         """Run one contact pass (mutating positions) and return its (i, j) pairs."""
         out = np.zeros(2 * cap, np.int32)
-        n = lib().oracle_contact_pairs_f32(C.byref(self.w), out.ctypes.data_as(C.POINTER(C.c_int32)), cap)
+        fn = getattr(lib(), f"oracle_contact_pairs_{self._sfx}")
+        n = fn(C.byref(self.w), out.ctypes.data_as(C.POINTER(C.c_int32)), cap)
         return out[: 2 * min(n, cap)].reshape(-1, 2)
 
     def bruteforce_rows(self, i0, i1):

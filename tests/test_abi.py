@@ -1,0 +1,182 @@
+"""The C ABI (include/blobs_gpu.h) on the CPU: exported symbols and the
+host-side construction path (arenas, mass setup, joints), which needs no
+GPU.  Compute entry points are not called here unless a device exists."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from blobs_amd import _lib as L
+from blobs_amd import scenes
+from blobs_amd.physics import ColliderBuilder, Physics, RigidBodyBuilder, Spring, STATIC
+from oracle import oracle
+
+from .conftest import gpu_available
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_functions():
+    text = (ROOT / "include" / "blobs_gpu.h").read_text()
+    return sorted(set(re.findall(r"^int (bg_\w+)\(", text, flags=re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = L.load()
+    names = header_functions()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in L.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors agree with the C compiler on sizeof/offsetof."""
+    import subprocess
+    structs = {"bg_rigid_body": L.RigidBodyC, "bg_collider": L.ColliderC, "bg_spring": L.SpringC,
+               "bg_fixed_joint": L.FixedJointC, "bg_params": L.ParamsC, "bg_body_batch": L.BodyBatchC,
+               "bg_collider_batch": L.ColliderBatchC}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "blobs_gpu.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                  check=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(out[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(out[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_physics_new_defaults():
+    """physics.rs:42-68."""
+    p = Physics((0.5, -2.0), False)
+    assert p.gravity == (0.5, -2.0)
+    assert p.substeps == 8 and p.joint_iterations == 4
+    assert p.collisions_enabled and not p.use_spatial_hash
+    assert p.time == 0.0 and p.accumulator == 0.0 and p.old_dt == 1.0
+
+
+def test_insert_collider_doubles_mass_and_matches_oracle():
+    """Q2 (collider.rs:179-181 + physics.rs:144) through the product's
+    construction path equals the oracle's restatement bit for bit."""
+    sc = scenes.cfg3_multi(n=256, n_springs=64, n_joints=32, steps=1)
+    p, bh = sc.build_physics()
+    m, I, com = oracle.mass_setup(sc)
+    for b in range(sc.n_bodies):
+        r = p.get_rbd(int(bh[b]))
+        assert np.float32(r.calculated_mass) == m[b]
+        assert np.float32(r.inertia) == I[b]
+        assert np.array_equal(np.array(r.center_of_mass, np.float32), com[b])
+
+
+def test_create_fixed_joint_distance_and_target_match_oracle():
+    sc = scenes.cfg3_multi(n=256, n_springs=8, n_joints=64, steps=1)
+    rot = np.linspace(-1, 1, sc.n_bodies).astype(np.float32)
+    sc.rotation = rot
+    sc.joint_anchor_a = np.full((64, 2), 0.25, np.float32)
+    sc.joint_anchor_b = np.full((64, 2), -0.5, np.float32)
+    p, bh = sc.build_physics()
+    dist, targ = oracle.joint_setup(sc)
+    lib = L.load()
+    # joint handles: slots 0..63 in insertion order, generation 1
+    for k in range(64):
+        j = L.FixedJointC()
+        L.check(lib.bg_get_joint(p._w, (1 << 32) | k, C.byref(j)), p._w)
+        assert np.float32(j.distance) == dist[k]
+        assert np.float32(j.target_angle) == targ[k]
+        assert j.rigid_body_a == int(bh[sc.joint_a[k]])
+
+
+def test_create_fixed_joint_on_one_body_panics():
+    p = Physics((0.0, 0.0), False)
+    h = p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    with pytest.raises(L.BlobsError) as e:
+        p.create_fixed_joint(h, h, (0, 0), (0, 0))
+    assert e.value.code == L.BG_ERR_PANIC
+
+
+def test_arena_handles_slot_reuse_and_iteration_order():
+    """thunderdome 0.6: LIFO free list, generation +1 on reuse, slot-order
+    iteration (generation numbering itself: parity unpinned)."""
+    p = Physics((0.0, 0.0), False)
+    hs = [p.insert_rbd(RigidBodyBuilder().position((float(k), 0)).build()) for k in range(4)]
+    assert hs == [(1 << 32) | k for k in range(4)]
+    p.remove_rbd(hs[1])
+    p.remove_rbd(hs[2])
+    assert p.rbd_count() == 2
+    assert p.get_rbd(hs[1]) is None
+    h_new = p.insert_rbd(RigidBodyBuilder().position((9.0, 0)).build())
+    assert h_new == (2 << 32) | 2  # last freed slot first
+    b = p.bodies()
+    assert b["handles"].tolist() == [hs[0], h_new, hs[3]]
+    assert b["position"][:, 0].tolist() == [0.0, 9.0, 3.0]
+
+
+def test_remove_last_collider_removes_body():
+    """ColliderSet::remove, collider.rs:134-164."""
+    p = Physics((0.0, 0.0), False)
+    h = p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    c1 = p.insert_collider_with_parent(ColliderBuilder().radius(0.5).build(), h)
+    c2 = p.insert_collider_with_parent(ColliderBuilder().radius(0.25).build(), h)
+    assert p.get_rbd(h).calculated_mass == 2 * (1.0 + 0.5)
+    p.remove_col(c1)
+    assert p.get_rbd(h).calculated_mass == 2 * 0.5
+    p.remove_col(c2)
+    assert p.get_rbd(h) is None and p.get_col(c2) is None
+
+
+def test_remove_rbd_removes_its_colliders():
+    p = Physics((0.0, 0.0), False)
+    h = p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    c = p.insert_collider_with_parent(ColliderBuilder().radius(0.5).build(), h)
+    p.remove_rbd(h)
+    assert p.get_col(c) is None and p.rbd_count() == 0
+
+
+def test_spring_and_constraint_arenas():
+    p = Physics((0.0, 0.0), False)
+    a = p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    b = p.insert_rbd(RigidBodyBuilder().position((1, 0)).build())
+    s = p.springs.insert(Spring(a, b, 1.0, 10.0, 0.5))
+    got = L.SpringC()
+    L.check(L.load().bg_get_spring(p._w, s, C.byref(got)), p._w)
+    assert (got.rigid_body_a, got.rigid_body_b, got.rest_length, got.stiffness, got.damping) == (a, b, 1.0, 10.0, 0.5)
+
+
+def test_spatial_hash_step_is_refused_like_the_reference():
+    """physics.rs:411-412 panics when use_spatial_hash and collisions are on."""
+    p = Physics((0.0, 0.0), True)
+    p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    with pytest.raises(L.BlobsError) as e:
+        p.step(1 / 60)
+    assert e.value.code == L.BG_ERR_UNSUPPORTED
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_step_without_device_fails_loudly():
+    p = Physics((0.0, -9.81), False)
+    p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    with pytest.raises(L.BlobsError) as e:
+        p.step(1 / 60)
+    assert e.value.code == L.BG_ERR_NO_DEVICE
+
+
+def test_rigid_body_builder_defaults():
+    """rigid_body.rs:303-318 and build() (:376-400)."""
+    b = RigidBodyBuilder().position((1, 2)).build()
+    assert b.position == b.position_old == (1.0, 2.0)
+    assert b.calculated_mass == 1.0 and b.inertia == 1.0 and b.gravity_mod == 1.0
+    assert b.velocity_request is None and b.body_type == 0
+    p = Physics((0.0, 0.0), False)
+    h = p.insert_rbd(b)
+    got = p.get_rbd(h)
+    assert got.position == (1.0, 2.0) and got.calculated_mass == 1.0
