@@ -83,6 +83,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rebuild-interval", type=int, default=None)
     ap.add_argument("--cell-scale", type=float, default=None)
+    ap.add_argument("--skin", type=float, default=None)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -98,8 +99,8 @@ def main() -> int:
     log(f"scene: {sc.n_bodies} spheres, box {sc.meta['box_side']:.1f}, forced overlaps "
         f"{sc.meta['forced_overlaps']} ({time.perf_counter() - t0:.1f} s)")
     p, _ = sc.build_physics()
-    if args.rebuild_interval or args.cell_scale:
-        p.set_broadphase(args.cell_scale or 1.0, args.rebuild_interval or 4)
+    if args.rebuild_interval or args.cell_scale or args.skin:
+        p.set_broadphase(args.cell_scale or 1.0, args.rebuild_interval or 4, args.skin or 0.125)
 
     # warmup (uploads the world, builds the grid)
     t0 = time.perf_counter()

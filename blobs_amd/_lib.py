@@ -164,7 +164,7 @@ SIGNATURES = {
     "bg_get_collider_positions": [WORLD, f32p, u32p],
     "bg_contact_count": [WORLD, u64p],
     "bg_last_error": [WORLD, C.c_char_p, C.c_size_t],
-    "bg_set_broadphase": [WORLD, C.c_float, C.c_uint32],
+    "bg_set_broadphase": [WORLD, C.c_float, C.c_float, C.c_uint32],
     "bg_step_path": [WORLD, u32p],
     "bg_set_profiling": [WORLD, C.c_int],
     "bg_stage_stats": [WORLD, C.c_uint32, u64p, C.POINTER(C.c_double)],

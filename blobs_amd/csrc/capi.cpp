@@ -192,8 +192,8 @@ int bg_last_error(bg_world* w, char* buf, size_t len) {
     return BG_OK;
 }
 
-int bg_set_broadphase(bg_world* w, float cell_scale, uint32_t rebuild_interval) {
-    return guarded(w, [&](bg::World& x) { return x.set_broadphase(cell_scale, rebuild_interval); });
+int bg_set_broadphase(bg_world* w, float cell_scale, float skin, uint32_t rebuild_interval) {
+    return guarded(w, [&](bg::World& x) { return x.set_broadphase(cell_scale, skin, rebuild_interval); });
 }
 
 int bg_step_path(bg_world* w, uint32_t* out) {

@@ -117,7 +117,8 @@ struct StepAccum {
     uint32_t dmax;      // f2ord(max |x - x0 - drift|), monotone within a rebuild period
     uint32_t pad0;
     uint32_t bbox[2][4];  // [parity]{minx, miny, maxx, maxy} as f2ord
-    unsigned long long contacts;
+    uint32_t n_outliers;  // sphere path: bodies beyond the skin since the last rebuild
+    uint32_t pad2;
     uint32_t error_flags;
     uint32_t push_outs;
     float old_dt;  // Physics::old_dt (physics.rs:33), world scalar

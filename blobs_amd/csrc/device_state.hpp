@@ -2,10 +2,11 @@
 // of the per-substep kernels.
 //
 // Structure of arrays, f32, indexed by "dense" index (arena slot order with
-// holes removed, i.e. the reference's iteration order).  The fused sphere
-// path additionally keeps bodies physically ordered by broadphase cell
-// (`aid[slot]` maps back to the dense index); the general path keeps dense
-// order and reaches colliders through the sorted index array.
+// holes removed, i.e. the reference's iteration order).  The general path
+// keeps dense order and reaches colliders through the sorted index array.
+// The fused sphere path keeps bodies physically ordered by broadphase cell
+// (`aid` in the snapshot maps a slot back to its dense index) in packed
+// 16-byte records.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,18 +33,42 @@ struct ColliderArrays {
     uint32_t *cflags;  // bit0 = sensor
 };
 
+// Fused sphere path: body and its only collider share a slot.
+//   snap.w bits: dense id (31 bits) | outlier flag for the next substep (bit 31)
+struct SphereArrays {
+    float4 *pp;    // {position.x, position.y, position_old.x, position_old.y}
+    float4 *snap;  // {absolute translation x, y, radius, bits(id | outlier << 31)}
+    float2 *vel;   // calculated_velocity
+    float2 *rw;    // {rotation, angular_velocity}
+    float2 *mg;    // {calculated_mass, gravity_mod}
+    uint32_t *bflags;
+    float2 *off;   // collider offset translation
+    uint2 *groups;
+    float *inertia, *torque;
+    float2 *acc, *vreq;
+};
+
+constexpr uint32_t kSnapOutlier = 0x80000000u;
+constexpr uint32_t kNil = 0xFFFFFFFFu;
+
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
-    uint32_t cap_b = 0, cap_c = 0, cap_s = 0, cap_j = 0, cap_k = 0, cap_bc = 0, cap_bs = 0, cap_jc = 0;
     uint32_t cells_cap = 0;
     int key_bits = 0;
 
-    BodyArrays b{}, b_alt{};
-    ColliderArrays c{}, c_alt{};
-    float2 *abs_next = nullptr;  // sphere path ping-pong snapshot
-    float2 *x0 = nullptr;        // collider snapshot at the last rebuild
-    // body -> colliders (general path)
-    uint32_t *bc_start = nullptr, *bc_list = nullptr;
+    // general path
+    BodyArrays b{};
+    ColliderArrays c{};
+    float2 *x0 = nullptr;  // collider snapshot at the last rebuild (sphere path: per slot)
+    uint32_t *bc_start = nullptr, *bc_list = nullptr;  // body -> colliders
+    // sphere path
+    SphereArrays s{}, s_alt{};
+    float4 *snap_next = nullptr;
+    // outlier cells: hashed linked lists of bodies that left their rebuild
+    // cell by more than the skin, rebuilt every substep (3 rotating tables)
+    uint32_t *oh_head[3] = {nullptr, nullptr, nullptr};
+    uint32_t *oh_next[3] = {nullptr, nullptr, nullptr};
+    uint32_t oh_size = 0;  // power of two
     // springs
     int32_t *sp_a = nullptr, *sp_b = nullptr;
     float *sp_rest = nullptr, *sp_k = nullptr, *sp_c = nullptr;
@@ -60,7 +85,6 @@ struct DeviceWorld {
     // broadphase
     uint32_t *keys = nullptr, *vals = nullptr, *keys_tmp = nullptr, *vals_tmp = nullptr;
     uint32_t *cell_start = nullptr;
-    uint32_t *slot_of = nullptr;  // sphere path: dense id -> slot
     GridParams *grid = nullptr;
     StepAccum *accum = nullptr;
     unsigned long long *block_contacts = nullptr;  // contact count per block of the per-body kernels
@@ -80,22 +104,23 @@ struct SubstepArgs {
     float rmax;         // largest collider radius
     float drift_prev_x, drift_prev_y;  // common drift at the previous snapshot (broadphase query centre)
     float drift_x, drift_y;            // common drift after this substep
+    float skin;         // sphere path: displacement since the rebuild beyond which a body is an outlier
     int32_t first_dyn;  // dense index of the first non-static body (Q1), -1 if none
     uint32_t fresh;     // first substep after an upload: user acc/torque are live
     uint32_t collisions;
     uint32_t parity;    // which bbox accumulator this substep writes
     uint32_t has_springs;
-    uint32_t inertia_all_pos;
+    uint32_t sub;       // sphere path: substeps since the last rebuild (1-based)
 };
 
-// Launchers (step_kernels.hip).
+// Launchers (step_kernels.hip: general path + broadphase build).
 hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity, bool sphere_path);
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d);
-hipError_t launch_sphere_permute(DeviceWorld& d, bool carry_user_inputs);
 hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t joint_iterations, bool groups,
                                   bool sensors);
+// Launchers (sphere_kernels.hip).
+hipError_t launch_sphere_permute(DeviceWorld& d, bool carry_user_inputs);
 hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool groups, bool sensors, bool zero_off,
-                                 float2* abs_in, float2* abs_out);
-hipError_t launch_sphere_unpermute(DeviceWorld& d, float2* abs_cur);
+                                 bool constraints);
 
 }  // namespace bg

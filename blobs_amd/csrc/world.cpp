@@ -478,10 +478,12 @@ int World::contact_count(uint64_t* out) {
     return BG_OK;
 }
 
-int World::set_broadphase(float cell_scale, uint32_t rebuild_interval) {
-    if (!(cell_scale >= 0.25f) || rebuild_interval == 0)
-        return fail(BG_ERR_INVALID_ARGUMENT, "cell_scale must be >= 0.25 and rebuild_interval >= 1");
+int World::set_broadphase(float cell_scale, float skin, uint32_t rebuild_interval) {
+    if (!(cell_scale >= 0.25f) || !(skin > 0.0f && skin <= 4.0f) || rebuild_interval == 0)
+        return fail(BG_ERR_INVALID_ARGUMENT,
+                    "cell_scale must be >= 0.25, skin in (0, 4] and rebuild_interval >= 1");
     cell_scale_ = cell_scale;
+    skin_scale_ = skin;
     rebuild_interval_ = rebuild_interval;
     need_rebuild_ = true;
     return BG_OK;
@@ -503,20 +505,25 @@ int World::ensure_device() {
 void World::free_arrays() {
     DeviceWorld& d = d_;
     if (d.stream) (void)hipStreamSynchronize(d.stream);
-    for (BodyArrays* b : {&d.b, &d.b_alt}) {
+    {
+        BodyArrays* b = &d.b;
         dfree(b->pos), dfree(b->pos_old), dfree(b->vel), dfree(b->acc), dfree(b->vreq);
         dfree(b->rot), dfree(b->omega), dfree(b->torque), dfree(b->inertia), dfree(b->mass), dfree(b->gmod);
         dfree(b->bflags), dfree(b->aid);
-    }
-    for (ColliderArrays* c : {&d.c, &d.c_alt}) {
+        ColliderArrays* c = &d.c;
         dfree(c->abs), dfree(c->off), dfree(c->radius), dfree(c->parent), dfree(c->groups), dfree(c->cflags);
     }
-    dfree(d.abs_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
+    for (SphereArrays* q : {&d.s, &d.s_alt}) {
+        dfree(q->pp), dfree(q->snap), dfree(q->vel), dfree(q->rw), dfree(q->mg), dfree(q->bflags), dfree(q->off);
+        dfree(q->groups), dfree(q->inertia), dfree(q->torque), dfree(q->acc), dfree(q->vreq);
+    }
+    for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]);
+    dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
     dfree(d.jc_start), dfree(d.jc_list), dfree(d.con_pos), dfree(d.con_r);
-    dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.slot_of);
+    dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start);
     dfree(d.grid), dfree(d.accum), dfree(d.block_contacts);
     sort_temp_free(d.sort);
     hipStream_t s = d.stream;
@@ -780,9 +787,56 @@ int World::upload() {
     d.cells_cap = next_pow2(std::max<uint32_t>(4u * nc, 1024u));
     d.key_bits = 0;
     while ((1u << d.key_bits) < d.cells_cap) ++d.key_bits;
-    const bool two = sphere;  // the sphere path permutes into a second copy
-    for (int copy = 0; copy < (two ? 2 : 1); ++copy) {
-        BodyArrays& b = copy ? d.b_alt : d.b;
+    hipStream_t s = d.stream;
+    if (sphere) {
+        // packed records in slot order; slot = dense index until the first rebuild
+        for (int copy = 0; copy < 2; ++copy) {
+            SphereArrays& q = copy ? d.s_alt : d.s;
+            HIP_TRY(dmalloc(&q.pp, nb));
+            HIP_TRY(dmalloc(&q.snap, nb));
+            HIP_TRY(dmalloc(&q.vel, nb));
+            HIP_TRY(dmalloc(&q.rw, nb));
+            HIP_TRY(dmalloc(&q.mg, nb));
+            HIP_TRY(dmalloc(&q.bflags, nb));
+            HIP_TRY(dmalloc(&q.off, nb));
+            HIP_TRY(dmalloc(&q.groups, nb));
+            HIP_TRY(dmalloc(&q.inertia, nb));
+            HIP_TRY(dmalloc(&q.torque, nb));
+            HIP_TRY(dmalloc(&q.acc, nb));
+            HIP_TRY(dmalloc(&q.vreq, nb));
+        }
+        HIP_TRY(dmalloc(&d.snap_next, nb));
+        d.oh_size = next_pow2(std::max<uint32_t>(1024u, nb / 4u));
+        for (int i = 0; i < 3; ++i) {
+            HIP_TRY(dmalloc(&d.oh_head[i], d.oh_size));
+            HIP_TRY(dmalloc(&d.oh_next[i], nb));
+        }
+        std::vector<float4> pp(nb), snap(nb);
+        std::vector<float2> rw(nb), mg(nb);
+        for (uint32_t i = 0; i < nb; ++i) {
+            pp[i] = make_float4(pos[i].x, pos[i].y, pos_old[i].x, pos_old[i].y);
+            uint32_t idbits = i;
+            float w;
+            memcpy(&w, &idbits, 4);
+            snap[i] = make_float4(cabs[i].x, cabs[i].y, crad[i], w);
+            rw[i] = make_float2(rot[i], omega[i]);
+            mg[i] = make_float2(mass[i], gmod[i]);
+        }
+        HIP_TRY(h2d(d.s.pp, pp, s));
+        HIP_TRY(h2d(d.s.snap, snap, s));
+        HIP_TRY(h2d(d.s.vel, vel, s));
+        HIP_TRY(h2d(d.s.rw, rw, s));
+        HIP_TRY(h2d(d.s.mg, mg, s));
+        HIP_TRY(h2d(d.s.bflags, bflags, s));
+        HIP_TRY(h2d(d.s.off, coff, s));
+        HIP_TRY(h2d(d.s.groups, cgrp, s));
+        HIP_TRY(h2d(d.s.inertia, inertia, s));
+        HIP_TRY(h2d(d.s.torque, torque, s));
+        HIP_TRY(h2d(d.s.acc, acc, s));
+        HIP_TRY(h2d(d.s.vreq, vreq, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        BodyArrays& b = d.b;
         HIP_TRY(dmalloc(&b.pos, nb));
         HIP_TRY(dmalloc(&b.pos_old, nb));
         HIP_TRY(dmalloc(&b.vel, nb));
@@ -796,15 +850,33 @@ int World::upload() {
         HIP_TRY(dmalloc(&b.gmod, nb));
         HIP_TRY(dmalloc(&b.bflags, nb));
         HIP_TRY(dmalloc(&b.aid, nb));
-        ColliderArrays& c = copy ? d.c_alt : d.c;
+        ColliderArrays& c = d.c;
         HIP_TRY(dmalloc(&c.abs, nc));
         HIP_TRY(dmalloc(&c.off, nc));
         HIP_TRY(dmalloc(&c.radius, nc));
         HIP_TRY(dmalloc(&c.parent, nc));
         HIP_TRY(dmalloc(&c.groups, nc));
         HIP_TRY(dmalloc(&c.cflags, nc));
+        HIP_TRY(h2d(d.b.pos, pos, s));
+        HIP_TRY(h2d(d.b.pos_old, pos_old, s));
+        HIP_TRY(h2d(d.b.vel, vel, s));
+        HIP_TRY(h2d(d.b.acc, acc, s));
+        HIP_TRY(h2d(d.b.vreq, vreq, s));
+        HIP_TRY(h2d(d.b.rot, rot, s));
+        HIP_TRY(h2d(d.b.omega, omega, s));
+        HIP_TRY(h2d(d.b.torque, torque, s));
+        HIP_TRY(h2d(d.b.inertia, inertia, s));
+        HIP_TRY(h2d(d.b.mass, mass, s));
+        HIP_TRY(h2d(d.b.gmod, gmod, s));
+        HIP_TRY(h2d(d.b.bflags, bflags, s));
+        HIP_TRY(h2d(d.b.aid, aid, s));
+        HIP_TRY(h2d(d.c.abs, cabs, s));
+        HIP_TRY(h2d(d.c.off, coff, s));
+        HIP_TRY(h2d(d.c.radius, crad, s));
+        HIP_TRY(h2d(d.c.parent, cpar, s));
+        HIP_TRY(h2d(d.c.groups, cgrp, s));
+        HIP_TRY(h2d(d.c.cflags, cfl, s));
     }
-    HIP_TRY(dmalloc(&d.abs_next, nc));
     HIP_TRY(dmalloc(&d.x0, nc));
     HIP_TRY(dmalloc(&d.bc_start, nb + 1));
     HIP_TRY(dmalloc(&d.bc_list, bc_list.size()));
@@ -830,7 +902,6 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.keys_tmp, nc));
     HIP_TRY(dmalloc(&d.vals_tmp, nc));
     HIP_TRY(dmalloc(&d.cell_start, (size_t)d.cells_cap + 1));
-    HIP_TRY(dmalloc(&d.slot_of, nb));
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
     d.n_block_slots = (std::max<uint32_t>(nb, 1) + 255) / 256;
@@ -840,27 +911,6 @@ int World::upload() {
     HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), d.stream));
     d.allocated = true;
 
-    // copy
-    hipStream_t s = d.stream;
-    HIP_TRY(h2d(d.b.pos, pos, s));
-    HIP_TRY(h2d(d.b.pos_old, pos_old, s));
-    HIP_TRY(h2d(d.b.vel, vel, s));
-    HIP_TRY(h2d(d.b.acc, acc, s));
-    HIP_TRY(h2d(d.b.vreq, vreq, s));
-    HIP_TRY(h2d(d.b.rot, rot, s));
-    HIP_TRY(h2d(d.b.omega, omega, s));
-    HIP_TRY(h2d(d.b.torque, torque, s));
-    HIP_TRY(h2d(d.b.inertia, inertia, s));
-    HIP_TRY(h2d(d.b.mass, mass, s));
-    HIP_TRY(h2d(d.b.gmod, gmod, s));
-    HIP_TRY(h2d(d.b.bflags, bflags, s));
-    HIP_TRY(h2d(d.b.aid, aid, s));
-    HIP_TRY(h2d(d.c.abs, cabs, s));
-    HIP_TRY(h2d(d.c.off, coff, s));
-    HIP_TRY(h2d(d.c.radius, crad, s));
-    HIP_TRY(h2d(d.c.parent, cpar, s));
-    HIP_TRY(h2d(d.c.groups, cgrp, s));
-    HIP_TRY(h2d(d.c.cflags, cfl, s));
     HIP_TRY(h2d(d.bc_start, bc_start, s));
     HIP_TRY(h2d(d.bc_list, bc_list, s));
     HIP_TRY(h2d(d.sp_a, sp_a, s));
@@ -910,17 +960,38 @@ int World::download() {
     }
     hipStream_t s = d.stream;
     const uint32_t nb = d.nb, nc = d.nc;
-    std::vector<float2> pos, pos_old, vel, cabs;
-    std::vector<float> rot, omega;
-    std::vector<uint32_t> bflags, aid;
-    HIP_TRY(d2h(pos, d.b.pos, nb, s));
-    HIP_TRY(d2h(pos_old, d.b.pos_old, nb, s));
-    HIP_TRY(d2h(vel, d.b.vel, nb, s));
-    HIP_TRY(d2h(rot, d.b.rot, nb, s));
-    HIP_TRY(d2h(omega, d.b.omega, nb, s));
-    HIP_TRY(d2h(bflags, d.b.bflags, nb, s));
-    HIP_TRY(d2h(aid, d.b.aid, nb, s));
-    HIP_TRY(d2h(cabs, d.c.abs, nc, s));
+    std::vector<float2> pos(nb), pos_old(nb), vel, cabs(nc);
+    std::vector<float> rot(nb), omega(nb);
+    std::vector<uint32_t> bflags, aid(nb);
+    if (path_ == StepPath::Sphere) {
+        std::vector<float4> pp, snap;
+        std::vector<float2> rw;
+        HIP_TRY(d2h(pp, d.s.pp, nb, s));
+        HIP_TRY(d2h(snap, d.s.snap, nb, s));
+        HIP_TRY(d2h(vel, d.s.vel, nb, s));
+        HIP_TRY(d2h(rw, d.s.rw, nb, s));
+        HIP_TRY(d2h(bflags, d.s.bflags, nb, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (uint32_t k = 0; k < nb; ++k) {
+            pos[k] = make_float2(pp[k].x, pp[k].y);
+            pos_old[k] = make_float2(pp[k].z, pp[k].w);
+            rot[k] = rw[k].x;
+            omega[k] = rw[k].y;
+            cabs[k] = make_float2(snap[k].x, snap[k].y);
+            uint32_t w;
+            memcpy(&w, &snap[k].w, 4);
+            aid[k] = w & ~kSnapOutlier;
+        }
+    } else {
+        HIP_TRY(d2h(pos, d.b.pos, nb, s));
+        HIP_TRY(d2h(pos_old, d.b.pos_old, nb, s));
+        HIP_TRY(d2h(vel, d.b.vel, nb, s));
+        HIP_TRY(d2h(rot, d.b.rot, nb, s));
+        HIP_TRY(d2h(omega, d.b.omega, nb, s));
+        HIP_TRY(d2h(bflags, d.b.bflags, nb, s));
+        HIP_TRY(d2h(cabs, d.c.abs, nc, s));
+        for (uint32_t k = 0; k < nb; ++k) aid[k] = k;
+    }
     StepAccum acc;
     HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -978,6 +1049,8 @@ int World::rebuild() {
     bbox_parity_ ^= 1u;
     drift_x_ = drift_y_ = vdrift_x_ = vdrift_y_ = 0.0;
     h_prev_ = 0.0;
+    sub_since_rebuild_ = 0;
+    skin_ = skin_scale_ * cell;
     need_rebuild_ = false;
     steps_since_rebuild_ = 0;
     return BG_OK;
@@ -1009,13 +1082,11 @@ int World::run_substep(float h, bool fresh) {
     a.collisions = params_.collisions_enabled ? 1u : 0u;
     a.parity = bbox_parity_;
     a.has_springs = d.ns > 0 ? 1u : 0u;
-    a.inertia_all_pos = 0;
+    a.skin = skin_;
+    a.sub = ++sub_since_rebuild_;
     if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
     if (path_ == StepPath::Sphere) {
-        HIP_TRY(launch_sphere_substep(d, a, groups_, sensors_, zero_off_, d.c.abs, d.abs_next));
-        float2* t = d.c.abs;
-        d.c.abs = d.abs_next;
-        d.abs_next = t;
+        HIP_TRY(launch_sphere_substep(d, a, groups_, sensors_, zero_off_, d.nk > 0));
     } else {
         HIP_TRY(launch_general_substep(d, a, params_.joint_iterations, groups_, sensors_));
     }

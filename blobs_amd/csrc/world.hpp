@@ -71,7 +71,7 @@ public:
     int get_bodies(float* pos, float* pos_old, float* vel, float* rot, float* omega, bg_handle* handles);
     int get_collider_positions(float* abs, uint32_t* count);
     int contact_count(uint64_t* out);
-    int set_broadphase(float cell_scale, uint32_t rebuild_interval);
+    int set_broadphase(float cell_scale, float skin, uint32_t rebuild_interval);
     int set_profiling(bool enable);
     int stage_stats(uint32_t stage, uint64_t* launches, double* total_ms);
     uint32_t rbd_count() const { return bodies_.len(); }
@@ -118,6 +118,9 @@ private:
     uint32_t steps_since_rebuild_ = 0;
     uint32_t bbox_parity_ = 0;
     float cell_scale_ = 1.0f;
+    float skin_scale_ = 0.125f;  // outlier skin as a fraction of the cell size
+    float skin_ = 0.0f;
+    uint32_t sub_since_rebuild_ = 0;
     uint32_t rebuild_interval_ = 4;
     // world properties fixed at upload
     float rmax_ = 0.0f;

@@ -499,8 +499,11 @@ class Physics:
         self._chk(self._lib.bg_get_collider_positions(self._w, L.fptr(out), C.byref(n)))
         return out
 
-    def set_broadphase(self, cell_scale: float = 1.0, rebuild_interval: int = 4) -> None:
-        self._chk(self._lib.bg_set_broadphase(self._w, float(cell_scale), int(rebuild_interval)))
+    def set_broadphase(self, cell_scale: float = 1.0, rebuild_interval: int = 4, skin: float = 0.125) -> None:
+        """Broadphase tuning (results never depend on it): cell size as a
+        multiple of 2*max radius, steps between grid rebuilds, and the skin
+        (fraction of a cell) beyond which a body is tracked as an outlier."""
+        self._chk(self._lib.bg_set_broadphase(self._w, float(cell_scale), float(skin), int(rebuild_interval)))
 
     def set_profiling(self, enable: bool = True) -> None:
         self._chk(self._lib.bg_set_profiling(self._w, int(bool(enable))))

@@ -220,10 +220,12 @@ int bg_last_error(bg_world *w, char *buf, size_t len);
 
 /* ---- tuning ---------------------------------------------------------- */
 
-/* Grid cell size as a multiple of 2*max collider radius (>= 1) and the
- * number of steps between broadphase rebuilds (>= 1).  Results do not
- * depend on either; only speed does. */
-int bg_set_broadphase(bg_world *w, float cell_scale, uint32_t rebuild_interval);
+/* Broadphase tuning: grid cell size as a multiple of 2*max collider radius
+ * (>= 0.25), the skin (fraction of a cell a body may drift from its
+ * rebuild position before it is tracked as an outlier; fused sphere path)
+ * and the number of steps between grid rebuilds (>= 1).  Results do not
+ * depend on any of them; only speed does. */
+int bg_set_broadphase(bg_world *w, float cell_scale, float skin, uint32_t rebuild_interval);
 /* Which device path the last step ran: 0 none, 1 general, 2 fused spheres. */
 int bg_step_path(bg_world *w, uint32_t *out);
 
