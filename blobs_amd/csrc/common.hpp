@@ -62,12 +62,18 @@ __device__ __forceinline__ float2 body_xform(float rot, float2 p, float2 off) {
     return make_float2(rx + p.x, ry + p.y);
 }
 
+// Out-of-line copy for rarely taken paths, so its f64 sincos does not
+// inflate the register allocation of the calling kernel.
+static __device__ __noinline__ float2 body_xform_slow(float rot, float2 p, float2 off) {
+    return body_xform(rot, p, off);
+}
+
 // Same for a collider whose offset translation is (0,0): R*0 is a signed
 // zero, and x + (+-0) == x for every x != 0, so only zero components need the
 // full evaluation.
 __device__ __forceinline__ float2 body_xform_zero_off(float rot, float2 p) {
     if (p.x != 0.0f && p.y != 0.0f) return p;
-    return body_xform(rot, p, make_float2(0.0f, 0.0f));
+    return body_xform_slow(rot, p, make_float2(0.0f, 0.0f));
 }
 
 // ---- wave64 / block reductions ------------------------------------------

@@ -50,6 +50,7 @@ struct SphereArrays {
 
 constexpr uint32_t kSnapOutlier = 0x80000000u;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
+constexpr uint32_t kExtra = 8;  // candidate-list slots per body (overflow: hash query)
 
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
@@ -69,6 +70,10 @@ struct DeviceWorld {
     uint32_t *oh_head[3] = {nullptr, nullptr, nullptr};
     uint32_t *oh_next[3] = {nullptr, nullptr, nullptr};
     uint32_t oh_size = 0;  // power of two
+    // per-body candidate lists filled by outliers for the next substep
+    // (2 rotating buffers of kExtra slots per body, plus counts)
+    uint32_t *xl_count[2] = {nullptr, nullptr};
+    uint32_t *xl_list[2] = {nullptr, nullptr};
     // springs
     int32_t *sp_a = nullptr, *sp_b = nullptr;
     float *sp_rest = nullptr, *sp_k = nullptr, *sp_c = nullptr;

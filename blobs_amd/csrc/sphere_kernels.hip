@@ -9,14 +9,18 @@
 // :360-366) is ping-ponged so a substep can read all snapshots while
 // writing its own.
 //
-// Broadphase between rebuilds: the sorted grid holds the snapshot of the
-// rebuild (x0).  A body whose snapshot has moved more than `skin` from x0
-// (after removing the common gravity drift) becomes an outlier for the rest
-// of the period: every substep re-inserts the outliers into a hashed cell
-// table of their current snapshots.  A query scans the sorted grid with
-// radius r + r_max + skin (skipping outliers) and the outlier table with
-// radius r + r_max, so every contact of the reference predicate is visited
-// exactly once, whatever the bodies do.
+// Broadphase between rebuilds.  The sorted grid holds the snapshots of the
+// rebuild (x0).  A body whose snapshot moves more than `skin` from x0 (after
+// removing the common gravity drift) is an outlier for the rest of the
+// period.  In the substep where a body is an outlier at the end, it
+//   - inserts itself into a hashed cell table of current snapshots, and
+//   - pushes its slot onto the candidate list of every body whose x0 cell is
+//     within r + r_max + skin of it (the bodies that can touch it next).
+// A regular body then visits: the sorted grid with radius r + r_max + skin
+// (skipping outliers) plus its candidate list; an outlier visits the sorted
+// grid plus the outlier table.  Every pair of the reference predicate is
+// visited exactly once from each side, whatever the bodies do; the skin and
+// rebuild interval change only the speed.
 #include "common.hpp"
 #include "device_state.hpp"
 #include "step_common.hpp"
@@ -38,6 +42,10 @@ struct SphereSubstep {
     uint32_t* oh_ins_next;
     uint32_t* oh_clr_head;                 // table cleared for the substep after next
     uint32_t oh_mask;
+    uint32_t* xl_count_in;                 // candidate lists read (and emptied) this substep
+    const uint32_t* __restrict__ xl_list_in;
+    uint32_t* xl_count_out;                // candidate lists filled for the next substep
+    uint32_t* xl_list_out;
     uint32_t nb;
     StepAccum* accum;
     unsigned long long* block_contacts;
@@ -54,6 +62,34 @@ __device__ __forceinline__ int32_t free_cell(float x, float o, float inv) {
 
 __device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, uint32_t mask) {
     return (((uint32_t)cx * 0x9E3779B1u) ^ ((uint32_t)cy * 0x85EBCA77u)) & mask;
+}
+
+// Rows of the sorted grid overlapping the square of half-width Q around
+// (qx, qy): calls row(t0, t1) with the slot range of each row.  Cell
+// coordinates are monotone in x, so the rows cover every x0 within Q.
+template <class F>
+__device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* __restrict__ cell_start, uint32_t n,
+                                          float qx, float qy, float Q, F&& row) {
+    const float R = Q + (1e-5f * (fabsf(qx) + fabsf(qy) + Q) + 1e-6f);
+    int32_t cx0, cx1, cy0, cy1;
+    if (R < 1e30f && finitef(qx) && finitef(qy)) {
+        cx0 = cell_coord(qx - R, g.ox, g.inv_cell, g.nx);
+        cx1 = cell_coord(qx + R, g.ox, g.inv_cell, g.nx);
+        cy0 = cell_coord(qy - R, g.oy, g.inv_cell, g.ny);
+        cy1 = cell_coord(qy + R, g.oy, g.inv_cell, g.ny);
+    } else {
+        cx0 = 0;
+        cx1 = g.nx - 1;
+        cy0 = 0;
+        cy1 = g.ny - 1;
+    }
+    for (int32_t cy = cy0; cy <= cy1; ++cy) {
+        const uint32_t rowc = (uint32_t)cy * (uint32_t)g.nx;
+        const uint32_t t0 = cell_start[rowc + (uint32_t)cx0];
+        uint32_t t1 = cell_start[rowc + (uint32_t)cx1 + 1u];
+        t1 = t1 < n ? t1 : n;
+        row(t0, t1);
+    }
 }
 
 template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS>
@@ -78,12 +114,13 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         const bool was_out = (wa & kSnapOutlier) != 0u;
         const float2 mgv = k.s.mg[s];
         const float mown = mgv.x;
+        const uint32_t xl_n = k.xl_count_in[s];
+        if (xl_n) k.xl_count_in[s] = 0u;
         uint32_t count = 0, pushes = 0, flags = 0;
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
             const float ra = own.z;
             const GridParams g = *k.grid;
-            const uint32_t nout = __hip_atomic_load(&k.accum->n_outliers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             uint2 ga = make_uint2(0, 0);
             if (GROUPS) ga = k.s.groups[s];
             const bool sen_a = SENSORS ? (fl & BF_SENSOR) != 0u : false;
@@ -105,21 +142,36 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                 sink(t, idb, isA, ax, ay, dist, rs);
             };
             auto enumerate = [&](auto&& sink) {
-                scan_stencil(g, k.cell_start, k.nb, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, ra + a.rmax + a.skin,
-                             [&](uint32_t t) {
-                                 if (t == s) return;
-                                 const float4 sb = k.snap_in[t];
-                                 if (__float_as_uint(sb.w) & kSnapOutlier) return;  // in the outlier table
-                                 test(t, sb, sink);
-                             });
-                if (nout == 0u) return;
+                // sorted grid: four candidates per load batch (clamped indices, no branches)
+                grid_rows(g, k.cell_start, k.nb, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, ra + a.rmax + a.skin,
+                          [&](uint32_t t0, uint32_t t1) {
+                              for (uint32_t t = t0; t < t1; t += 4) {
+                                  float4 sb[4];
+#pragma unroll
+                                  for (int u = 0; u < 4; ++u) sb[u] = k.snap_in[min(t + u, t1 - 1)];
+#pragma unroll
+                                  for (int u = 0; u < 4; ++u) {
+                                      const uint32_t tu = t + u;
+                                      if (tu < t1 && tu != s && !(__float_as_uint(sb[u].w) & kSnapOutlier))
+                                          test(tu, sb[u], sink);
+                                  }
+                              }
+                          });
+                if (!was_out && xl_n <= kExtra) {
+                    // outliers that pushed themselves onto this body's list
+                    for (uint32_t i = 0; i < xl_n; ++i) {
+                        const uint32_t t = k.xl_list_in[s * kExtra + i];
+                        test(t, k.snap_in[t], sink);
+                    }
+                    return;
+                }
+                // outlier body, or a list that overflowed: the outlier table
                 float R = ra + a.rmax;
                 R = R + (1e-5f * (fabsf(xa.x) + fabsf(xa.y) + R) + 1e-6f);
-                int32_t cx0 = free_cell(xa.x - R, g.ox, g.inv_cell), cx1 = free_cell(xa.x + R, g.ox, g.inv_cell);
-                int32_t cy0 = free_cell(xa.y - R, g.oy, g.inv_cell), cy1 = free_cell(xa.y + R, g.oy, g.inv_cell);
+                const int32_t cx0 = free_cell(xa.x - R, g.ox, g.inv_cell), cx1 = free_cell(xa.x + R, g.ox, g.inv_cell);
+                const int32_t cy0 = free_cell(xa.y - R, g.oy, g.inv_cell), cy1 = free_cell(xa.y + R, g.oy, g.inv_cell);
                 if (!(R < 1e30f) || cx1 - cx0 > 64 || cy1 - cy0 > 64) {
-                    // huge radius: walk every bucket once (superset)
-                    for (uint32_t h = 0; h <= k.oh_mask; ++h)
+                    for (uint32_t h = 0; h <= k.oh_mask; ++h)  // huge radius: every bucket (superset)
                         for (uint32_t t = k.oh_head[h]; t != kNil; t = k.oh_next[t])
                             if (t != s) test(t, k.snap_in[t], sink);
                     return;
@@ -242,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         }
         k.s.vel[s] = vel;
         // collider transform from the pre-constraint position (physics.rs:360-366)
-        const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform(st.rot, st.p, k.s.off[s]);
+        const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform_slow(st.rot, st.p, k.s.off[s]);
         const bool finite_x = finitef(x.x) && finitef(x.y);
         bool out = was_out;
         if (!out && finite_x) {
@@ -252,11 +304,21 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             new_outliers += out ? 1u : 0u;
         }
         k.snap_out[s] = make_float4(x.x, x.y, own.z, __uint_as_float(id | (out ? kSnapOutlier : 0u)));
-        if (out && finite_x) {
+        if (out && finite_x && a.collisions) {
             const GridParams g = *k.grid;
+            // outlier table of the next substep
             const uint32_t h = cell_hash(free_cell(x.x, g.ox, g.inv_cell), free_cell(x.y, g.oy, g.inv_cell), k.oh_mask);
             const uint32_t old = atomicExch(&k.oh_ins_head[h], s);
             k.oh_ins_next[s] = old;
+            // every body whose rebuild cell is within reach gets this slot on its list
+            grid_rows(g, k.cell_start, k.nb, x.x - a.drift_x, x.y - a.drift_y, own.z + a.rmax + a.skin,
+                      [&](uint32_t t0, uint32_t t1) {
+                          for (uint32_t t = t0; t < t1; ++t) {
+                              if (t == s) continue;
+                              const uint32_t c = atomicAdd(&k.xl_count_out[t], 1u);
+                              if (c < kExtra) k.xl_list_out[t * kExtra + c] = s;
+                          }
+                      });
         }
         la.point(x.x, x.y, x.x, x.y, 0.0f, 0.0f);
         if (CONSTRAINTS) apply_constraints(st.p, k.con_pos, k.con_r, k.nk);
@@ -283,16 +345,20 @@ struct SpherePermute {
     float2* __restrict__ x0;
     uint32_t* heads[3];
     uint32_t oh_size;
+    uint32_t* xl_count[2];
     uint32_t nb;
     uint32_t carry_user;
 };
 
 // Gather every per-body record into cell order: dst[s] = src[vals[s]]; the
-// new snapshot is also the new x0, and no body is an outlier.
+// new snapshot is also the new x0, no body is an outlier, the outlier tables
+// and candidate lists are empty.
 __global__ __launch_bounds__(kThreads) void k_sphere_permute(SpherePermute k) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = s; i < 3u * k.oh_size; i += gridDim.x * blockDim.x) k.heads[i / k.oh_size][i % k.oh_size] = kNil;
     if (s >= k.nb) return;
+    k.xl_count[0][s] = 0u;
+    k.xl_count[1][s] = 0u;
     const uint32_t o = k.vals[s];
     if (o >= k.nb) return;  // a permutation of [0, nb) by construction
     k.dst.pp[s] = k.src.pp[o];
@@ -326,6 +392,8 @@ hipError_t launch_sphere_permute(DeviceWorld& d, bool carry_user_inputs) {
     k.x0 = d.x0;
     for (int i = 0; i < 3; ++i) k.heads[i] = d.oh_head[i];
     k.oh_size = d.oh_size;
+    k.xl_count[0] = d.xl_count[0];
+    k.xl_count[1] = d.xl_count[1];
     k.nb = d.nb;
     k.carry_user = carry_user_inputs ? 1u : 0u;
     k_sphere_permute<<<div_up(d.nb, kThreads), kThreads, 0, d.stream>>>(k);
@@ -353,6 +421,10 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.oh_ins_next = d.oh_next[(j + 1) % 3];
     k.oh_clr_head = d.oh_head[(j + 2) % 3];
     k.oh_mask = d.oh_size - 1;
+    k.xl_count_in = d.xl_count[j % 2];
+    k.xl_list_in = d.xl_list[j % 2];
+    k.xl_count_out = d.xl_count[(j + 1) % 2];
+    k.xl_list_out = d.xl_list[(j + 1) % 2];
     k.nb = d.nb;
     k.accum = d.accum;
     k.block_contacts = d.block_contacts;

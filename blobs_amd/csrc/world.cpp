@@ -518,6 +518,7 @@ void World::free_arrays() {
         dfree(q->groups), dfree(q->inertia), dfree(q->torque), dfree(q->acc), dfree(q->vreq);
     }
     for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]);
+    for (int i = 0; i < 2; ++i) dfree(d.xl_count[i]), dfree(d.xl_list[i]);
     dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
@@ -810,6 +811,11 @@ int World::upload() {
         for (int i = 0; i < 3; ++i) {
             HIP_TRY(dmalloc(&d.oh_head[i], d.oh_size));
             HIP_TRY(dmalloc(&d.oh_next[i], nb));
+        }
+        for (int i = 0; i < 2; ++i) {
+            HIP_TRY(dmalloc(&d.xl_count[i], nb));
+            HIP_TRY(dmalloc(&d.xl_list[i], (size_t)nb * kExtra));
+            HIP_TRY(hipMemsetAsync(d.xl_count[i], 0, sizeof(uint32_t) * nb, s));
         }
         std::vector<float4> pp(nb), snap(nb);
         std::vector<float2> rw(nb), mg(nb);
