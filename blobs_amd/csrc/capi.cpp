@@ -204,6 +204,12 @@ int bg_step_path(bg_world* w, uint32_t* out) {
     });
 }
 
+// Internal diagnostic hook (bgx_: not part of the Physics ABI): the world's
+// step record {n_outliers, dmax bits, push_outs, error_flags}.
+int bgx_world_accum(bg_world* w, uint32_t* out4) {
+    return guarded(w, [&](bg::World& x) { return x.debug_accum(out4); });
+}
+
 int bg_set_profiling(bg_world* w, int enable) {
     return guarded(w, [&](bg::World& x) { return x.set_profiling(enable != 0); });
 }

@@ -104,6 +104,19 @@ int World::set_profiling(bool enable) {
     return BG_OK;
 }
 
+int World::debug_accum(uint32_t* out4) {
+    out4[0] = out4[1] = out4[2] = out4[3] = 0;
+    if (!d_.accum) return BG_OK;
+    StepAccum acc;
+    HIP_TRY(hipMemcpyAsync(&acc, d_.accum, sizeof(acc), hipMemcpyDeviceToHost, d_.stream));
+    HIP_TRY(hipStreamSynchronize(d_.stream));
+    out4[0] = acc.n_outliers;
+    out4[1] = acc.dmax;
+    out4[2] = acc.push_outs;
+    out4[3] = acc.error_flags;
+    return BG_OK;
+}
+
 int World::stage_stats(uint32_t stage, uint64_t* launches, double* total_ms) {
     if (stage >= BG_STAGE_COUNT) return fail(BG_ERR_INVALID_ARGUMENT, "unknown stage");
     int st = collect_stage_times();

@@ -73,6 +73,7 @@ public:
     int contact_count(uint64_t* out);
     int set_broadphase(float cell_scale, float skin, uint32_t rebuild_interval);
     int set_profiling(bool enable);
+    int debug_accum(uint32_t* out4);
     int stage_stats(uint32_t stage, uint64_t* launches, double* total_ms);
     uint32_t rbd_count() const { return bodies_.len(); }
     StepPath path() const { return path_; }
