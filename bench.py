@@ -100,7 +100,7 @@ def main() -> int:
         f"{sc.meta['forced_overlaps']} ({time.perf_counter() - t0:.1f} s)")
     p, _ = sc.build_physics()
     if args.rebuild_interval or args.cell_scale or args.skin:
-        p.set_broadphase(args.cell_scale or 1.0, args.rebuild_interval or 4, args.skin or 0.125)
+        p.set_broadphase(args.cell_scale or 1.0, args.rebuild_interval or 1, args.skin or 0.5)
 
     # warmup (uploads the world, builds the grid)
     t0 = time.perf_counter()

@@ -50,7 +50,6 @@ struct SphereArrays {
 
 constexpr uint32_t kSnapOutlier = 0x80000000u;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
-constexpr uint32_t kExtra = 8;  // candidate-list slots per body (overflow: hash query)
 
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
@@ -65,15 +64,18 @@ struct DeviceWorld {
     // sphere path
     SphereArrays s{}, s_alt{};
     float4 *snap_next = nullptr;
-    // outlier cells: hashed linked lists of bodies that left their rebuild
-    // cell by more than the skin, rebuilt every substep (3 rotating tables)
+    // outlier cells: hashed linked lists (plus an occupancy bitmap) of the
+    // bodies that moved more than the skin since the last rebuild, refilled
+    // every substep (3 rotating tables)
     uint32_t *oh_head[3] = {nullptr, nullptr, nullptr};
     uint32_t *oh_next[3] = {nullptr, nullptr, nullptr};
-    uint32_t oh_size = 0;  // power of two
-    // per-body candidate lists filled by outliers for the next substep
-    // (2 rotating buffers of kExtra slots per body, plus counts)
-    uint32_t *xl_count[2] = {nullptr, nullptr};
-    uint32_t *xl_list[2] = {nullptr, nullptr};
+    uint32_t *oh_bits[3] = {nullptr, nullptr, nullptr};
+    uint32_t oh_size = 0;  // power of two, >= 32
+    // rebuild: bodies per cell, and the slot -> previous slot permutation the
+    // first substep after a rebuild gathers its records through
+    uint32_t *cell_count = nullptr;
+    uint32_t *perm = nullptr;
+    bool gather_pending = false;
     // springs
     int32_t *sp_a = nullptr, *sp_b = nullptr;
     float *sp_rest = nullptr, *sp_k = nullptr, *sp_c = nullptr;
@@ -119,12 +121,12 @@ struct SubstepArgs {
 };
 
 // Launchers (step_kernels.hip: general path + broadphase build).
-hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity, bool sphere_path);
+hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity);
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d);
 hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t joint_iterations, bool groups,
                                   bool sensors);
 // Launchers (sphere_kernels.hip).
-hipError_t launch_sphere_permute(DeviceWorld& d, bool carry_user_inputs);
+hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity);
 hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool groups, bool sensors, bool zero_off,
                                  bool constraints);
 

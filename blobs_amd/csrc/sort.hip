@@ -1,8 +1,9 @@
 // sort.hip -- exclusive scan and stable LSD radix sort for gfx950.
 //
-// Scan: one launch, decoupled look-back (tiles take a ticket in launch order,
-// publish their aggregate, then walk back over predecessors until an
-// inclusive prefix is found).  Radix sort: 8-bit digits; per pass a digit
+// Scan: three launches with no inter-block waiting -- per-tile sums, one
+// block scanning the tile sums, then the per-tile scan offset by its tile's
+// prefix.  The input is read twice; nothing spins on another block's flag,
+// so the cost is bandwidth and launch latency only.  Radix sort: 8-bit digits; per pass a digit
 // histogram per 4096-item tile, the scan above over the digit-major matrix,
 // and a scatter that ranks items with wave64 ballots (peer masks from 8
 // ballots, popcount of lower lanes), orders the tile in LDS and writes
@@ -14,71 +15,79 @@ namespace bg {
 
 namespace {
 
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagIncl = 2ull << 62;
-
-__global__ void k_fill_u64(uint64_t* p, uint32_t n, uint64_t v) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ __launch_bounds__(kSortThreads) void k_scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                                 uint64_t* status, uint32_t ntiles) {
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
-    __shared__ uint32_t s_prefix;
+// Block-wide exclusive prefix of one value per thread; *total gets the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
     const int tid = threadIdx.x;
     const int lane = tid & (BG_WAVE - 1);
     const int wid = tid / BG_WAVE;
-    if (tid == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(&status[ntiles]), 1u);
+    const uint32_t incl = wave_incl_scan_u32(v);
+    if (lane == BG_WAVE - 1) s_wave[wid] = incl;
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t base = tile * kScanTile + tid * kSortItems;
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / BG_WAVE; ++w) {
+        off += (w < wid) ? s_wave[w] : 0u;
+        tot += s_wave[w];
+    }
+    *total = tot;
+    return off + incl - v;
+}
+
+// sums[tile] = sum of the tile's items (coalesced: item k * 256 + tid).
+__global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
+                                                            uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = base + k * kSortThreads;
+        sum += (i < n) ? in[i] : 0u;
+    }
+    sum = wave_sum_u32(sum);
+    if ((threadIdx.x & (BG_WAVE - 1)) == 0) s_wave[threadIdx.x / BG_WAVE] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kSortThreads / BG_WAVE; ++w) t += s_wave[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// In-place exclusive scan of the tile sums by one block.
+__global__ __launch_bounds__(kSortThreads) void k_scan_tile_sums(uint32_t* sums, uint32_t ntiles) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += kSortThreads) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < ntiles ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, s_wave, &total);
+        if (i < ntiles) sums[i] = carry + ex;
+        carry += total;
+        __syncthreads();  // s_wave is reused by the next chunk
+    }
+}
+
+// out = exclusive scan of in; thread t owns items [t*16, t*16+16) of its tile.
+__global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                             const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kSortItems;
     uint32_t v[kSortItems];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        uint32_t i = base + k;
+        const uint32_t i = base + k;
         v[k] = (i < n) ? in[i] : 0u;
         sum += v[k];
     }
-    uint32_t incl = wave_incl_scan_u32(sum);
-    if (lane == BG_WAVE - 1) s_wave[wid] = incl;
-    __syncthreads();
-    uint32_t wave_off = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kSortThreads / BG_WAVE; ++w) {
-        if (w < wid) wave_off += s_wave[w];
-        total += s_wave[w];
-    }
-    if (tid == 0) {
-        uint32_t excl = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&status[0], kFlagIncl | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&status[tile], kFlagAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int32_t j = (int32_t)tile - 1;
-            while (j >= 0) {
-                uint64_t st = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                uint64_t flag = st & (3ull << 62);
-                if (flag == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)st;
-                if (flag == kFlagIncl) break;
-                --j;
-            }
-            __hip_atomic_store(&status[tile], kFlagIncl | (uint32_t)(excl + total), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_prefix = excl;
-    }
-    __syncthreads();
-    uint32_t run = s_prefix + wave_off + (incl - sum);
+    uint32_t total;
+    uint32_t run = sums[blockIdx.x] + block_excl_scan(sum, s_wave, &total);
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        uint32_t i = base + k;
+        const uint32_t i = base + k;
         if (i < n) out[i] = run;
         run += v[k];
     }
@@ -207,12 +216,12 @@ hipError_t sort_temp_reserve(SortTemp& t, uint32_t max_items) {
     t.max_scan_tiles = div_up(scan_items, kScanTile);
     hipError_t e = hipMalloc(&t.hist, sizeof(uint32_t) * 256u * t.max_blocks);
     if (e != hipSuccess) return e;
-    return hipMalloc(&t.scan_status, sizeof(uint64_t) * (t.max_scan_tiles + 1));
+    return hipMalloc(&t.tile_sums, sizeof(uint32_t) * t.max_scan_tiles);
 }
 
 void sort_temp_free(SortTemp& t) {
     if (t.hist) (void)hipFree(t.hist);
-    if (t.scan_status) (void)hipFree(t.scan_status);
+    if (t.tile_sums) (void)hipFree(t.tile_sums);
     t = SortTemp{};
 }
 
@@ -220,8 +229,9 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, Sor
     if (n == 0) return hipSuccess;
     uint32_t ntiles = div_up(n, kScanTile);
     if (ntiles > t.max_scan_tiles) return hipErrorInvalidValue;
-    k_fill_u64<<<div_up(ntiles + 1, 256), 256, 0, s>>>(t.scan_status, ntiles + 1, 0ull);
-    k_scan_exclusive<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.scan_status, ntiles);
+    k_tile_sums<<<ntiles, kSortThreads, 0, s>>>(in, n, t.tile_sums);
+    k_scan_tile_sums<<<1, kSortThreads, 0, s>>>(t.tile_sums, ntiles);
+    k_scan_tiles<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.tile_sums);
     return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
-// sort.hpp -- device-wide primitives for the broadphase: a single-pass
-// exclusive scan (decoupled look-back) and a stable LSD radix sort of
+// sort.hpp -- device-wide primitives for the broadphase: an exclusive scan
+// (reduce, scan of tile sums, scan) and a stable LSD radix sort of
 // (key, value) u32 pairs with wave64 ballot ranking.
 #pragma once
 
@@ -10,7 +10,7 @@ namespace bg {
 
 struct SortTemp {
     uint32_t* hist = nullptr;      // [256 * max_blocks]
-    uint64_t* scan_status = nullptr;  // [max_scan_tiles + 1]
+    uint32_t* tile_sums = nullptr;  // [max_scan_tiles]
     uint32_t max_items = 0;
     uint32_t max_blocks = 0;
     uint32_t max_scan_tiles = 0;

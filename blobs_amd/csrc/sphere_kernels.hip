@@ -3,24 +3,25 @@
 // scene): gravity, contacts, Verlet, collider transform and circle
 // constraints of physics.rs:401-421 in ONE kernel per substep.
 //
-// Layout: bodies sit in broadphase-cell order (permuted at every rebuild),
-// in packed 16-byte records; the collider snapshot the contact pass reads
-// (physics.rs:264, written by the previous substep's update_objects at
-// :360-366) is ping-ponged so a substep can read all snapshots while
-// writing its own.
+// Layout: bodies sit in broadphase-cell order in packed 16-byte records; the
+// collider snapshot the contact pass reads (physics.rs:264, written by the
+// previous substep's update_objects at :360-366) is ping-ponged so a
+// substep can read every snapshot while writing its own.
 //
-// Broadphase between rebuilds.  The sorted grid holds the snapshots of the
-// rebuild (x0).  A body whose snapshot moves more than `skin` from x0 (after
-// removing the common gravity drift) is an outlier for the rest of the
-// period.  In the substep where a body is an outlier at the end, it
-//   - inserts itself into a hashed cell table of current snapshots, and
-//   - pushes its slot onto the candidate list of every body whose x0 cell is
-//     within r + r_max + skin of it (the bodies that can touch it next).
-// A regular body then visits: the sorted grid with radius r + r_max + skin
-// (skipping outliers) plus its candidate list; an outlier visits the sorted
-// grid plus the outlier table.  Every pair of the reference predicate is
-// visited exactly once from each side, whatever the bodies do; the skin and
-// rebuild interval change only the speed.
+// Broadphase.  Every step starts with a one-pass counting sort of the
+// snapshots by grid cell (cell-hash keys, atomic ranks, a decoupled
+// look-back scan for the cell table, a scatter of the snapshot records and
+// of the permutation).  The first substep after it gathers the body records
+// through the permutation and writes them back in cell order, so the
+// reordering costs no separate pass.  Between rebuilds the sorted grid keeps
+// the rebuild snapshots (x0); a body whose snapshot moves more than `skin`
+// from x0 (after removing the common gravity drift) becomes an outlier and
+// is, from then on, re-inserted every substep into a coarse hashed cell
+// table with an occupancy bitmap.  A query scans the sorted grid with radius
+// r + r_max + skin (skipping outliers) and the outlier table with radius
+// r + r_max, so every pair of the reference predicate is visited exactly
+// once from each side, whatever the bodies do; skin, cell size and rebuild
+// interval change only the speed.
 #include "common.hpp"
 #include "device_state.hpp"
 #include "step_common.hpp"
@@ -29,8 +30,12 @@ namespace bg {
 
 namespace {
 
+constexpr int kCoarseShift = 2;  // outlier table cells are 4x4 grid cells
+
 struct SphereSubstep {
-    SphereArrays s;
+    SphereArrays src;  // body records read (through perm on the first substep after a rebuild)
+    SphereArrays dst;  // body records written, slot order
+    const uint32_t* __restrict__ perm;  // new slot -> old slot (first substep after a rebuild)
     const float4* __restrict__ snap_in;
     float4* __restrict__ snap_out;
     const float2* __restrict__ x0;
@@ -38,14 +43,13 @@ struct SphereSubstep {
     const GridParams* __restrict__ grid;
     const uint32_t* __restrict__ oh_head;  // outlier table read this substep
     const uint32_t* __restrict__ oh_next;
+    const uint32_t* __restrict__ oh_bits;
     uint32_t* oh_ins_head;                 // table this substep fills (read by the next one)
     uint32_t* oh_ins_next;
+    uint32_t* oh_ins_bits;
     uint32_t* oh_clr_head;                 // table cleared for the substep after next
+    uint32_t* oh_clr_bits;
     uint32_t oh_mask;
-    uint32_t* xl_count_in;                 // candidate lists read (and emptied) this substep
-    const uint32_t* __restrict__ xl_list_in;
-    uint32_t* xl_count_out;                // candidate lists filled for the next substep
-    uint32_t* xl_list_out;
     uint32_t nb;
     StepAccum* accum;
     unsigned long long* block_contacts;
@@ -92,19 +96,22 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
     }
 }
 
-template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS>
+template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER>
 __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
     LocalAccum la;
     la.init();
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kThreads + threadIdx.x;
     // clear the outlier table of the substep after next
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i <= k.oh_mask; i += gridDim.x * kThreads)
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i <= k.oh_mask; i += gridDim.x * kThreads) {
         k.oh_clr_head[i] = kNil;
+        if ((i & 31u) == 0u) k.oh_clr_bits[i >> 5] = 0u;
+    }
     uint32_t new_outliers = 0;
     if (s < k.nb) {
-        const uint32_t fl = k.s.bflags[s];
-        const float4 pp = k.s.pp[s];
+        const uint32_t g_own = GATHER ? k.perm[s] : s;
+        const uint32_t fl = k.src.bflags[g_own];
+        const float4 pp = k.src.pp[g_own];
         BodyState st;
         st.p = make_float2(pp.x, pp.y);
         st.po = make_float2(pp.z, pp.w);
@@ -112,22 +119,21 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         const uint32_t wa = __float_as_uint(own.w);
         const uint32_t id = wa & ~kSnapOutlier;
         const bool was_out = (wa & kSnapOutlier) != 0u;
-        const float2 mgv = k.s.mg[s];
+        const float2 mgv = k.src.mg[g_own];
         const float mown = mgv.x;
-        const uint32_t xl_n = k.xl_count_in[s];
-        if (xl_n) k.xl_count_in[s] = 0u;
         uint32_t count = 0, pushes = 0, flags = 0;
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
             const float ra = own.z;
             const GridParams g = *k.grid;
             uint2 ga = make_uint2(0, 0);
-            if (GROUPS) ga = k.s.groups[s];
+            if (GROUPS) ga = k.src.groups[g_own];
             const bool sen_a = SENSORS ? (fl & BF_SENSOR) != 0u : false;
+            const uint32_t nout = __hip_atomic_load(&k.accum->n_outliers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // reference predicate (physics.rs:255-269) against the slot-t snapshot
             auto test = [&](uint32_t t, const float4 sb, auto&& sink) {
                 if (GROUPS) {
-                    const uint2 gb = k.s.groups[t];
+                    const uint2 gb = k.src.groups[GATHER ? k.perm[t] : t];
                     if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) return;
                 }
                 const uint32_t idb = __float_as_uint(sb.w) & ~kSnapOutlier;
@@ -157,19 +163,14 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                                   }
                               }
                           });
-                if (!was_out && xl_n <= kExtra) {
-                    // outliers that pushed themselves onto this body's list
-                    for (uint32_t i = 0; i < xl_n; ++i) {
-                        const uint32_t t = k.xl_list_in[s * kExtra + i];
-                        test(t, k.snap_in[t], sink);
-                    }
-                    return;
-                }
-                // outlier body, or a list that overflowed: the outlier table
+                if (nout == 0u) return;
+                // outlier table: coarse cells around the current snapshot
                 float R = ra + a.rmax;
                 R = R + (1e-5f * (fabsf(xa.x) + fabsf(xa.y) + R) + 1e-6f);
-                const int32_t cx0 = free_cell(xa.x - R, g.ox, g.inv_cell), cx1 = free_cell(xa.x + R, g.ox, g.inv_cell);
-                const int32_t cy0 = free_cell(xa.y - R, g.oy, g.inv_cell), cy1 = free_cell(xa.y + R, g.oy, g.inv_cell);
+                const int32_t cx0 = free_cell(xa.x - R, g.ox, g.inv_cell) >> kCoarseShift;
+                const int32_t cx1 = free_cell(xa.x + R, g.ox, g.inv_cell) >> kCoarseShift;
+                const int32_t cy0 = free_cell(xa.y - R, g.oy, g.inv_cell) >> kCoarseShift;
+                const int32_t cy1 = free_cell(xa.y + R, g.oy, g.inv_cell) >> kCoarseShift;
                 if (!(R < 1e30f) || cx1 - cx0 > 64 || cy1 - cy0 > 64) {
                     for (uint32_t h = 0; h <= k.oh_mask; ++h)  // huge radius: every bucket (superset)
                         for (uint32_t t = k.oh_head[h]; t != kNil; t = k.oh_next[t])
@@ -177,15 +178,19 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                     return;
                 }
                 for (int32_t cy = cy0; cy <= cy1; ++cy)
-                    for (int32_t cx = cx0; cx <= cx1; ++cx)
-                        for (uint32_t t = k.oh_head[cell_hash(cx, cy, k.oh_mask)]; t != kNil; t = k.oh_next[t]) {
+                    for (int32_t cx = cx0; cx <= cx1; ++cx) {
+                        const uint32_t h = cell_hash(cx, cy, k.oh_mask);
+                        if (!((k.oh_bits[h >> 5] >> (h & 31u)) & 1u)) continue;
+                        for (uint32_t t = k.oh_head[h]; t != kNil; t = k.oh_next[t]) {
                             if (t == s) continue;
                             const float4 sb = k.snap_in[t];
-                            // each outlier lives in exactly one cell: drop hash collisions
-                            if (free_cell(sb.x, g.ox, g.inv_cell) != cx || free_cell(sb.y, g.oy, g.inv_cell) != cy)
+                            // each outlier lives in one coarse cell: drop hash collisions
+                            if ((free_cell(sb.x, g.ox, g.inv_cell) >> kCoarseShift) != cx ||
+                                (free_cell(sb.y, g.oy, g.inv_cell) >> kCoarseShift) != cy)
                                 continue;
                             test(t, sb, sink);
                         }
+                    }
             };
             // correction of one contact for this body's side (physics.rs:272-300);
             // a push-out is folded into the same entry (order-dependent in the
@@ -204,12 +209,13 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                         flags |= INFO_PUSH_OUT;
                     }
                 }
-                const bool sen_b = SENSORS ? (k.s.bflags[t] & BF_SENSOR) != 0u : false;
+                const uint32_t tp = GATHER ? k.perm[t] : t;
+                const bool sen_b = SENSORS ? (k.src.bflags[tp] & BF_SENSOR) != 0u : false;
                 if (sen_a || sen_b) {
                     if (pushed) put(idb, px, py);
                     return;
                 }
-                const float mpart = k.s.mg[t].x;
+                const float mpart = k.src.mg[tp].x;
                 const float mA = isA ? mown : mpart, mB = isA ? mpart : mown;
                 const float ratio = 1.0f - mA / (mA + mB);
                 const float nx = ax / dist, ny = ay / dist;
@@ -265,21 +271,24 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             }
         }
         // update_objects (physics.rs:326-358)
-        const float2 rw = k.s.rw[s];
+        const float2 rw = k.src.rw[g_own];
         st.rot = rw.x;
         st.omega = rw.y;
         float2 vel;
+        uint32_t fl_out = fl;
+        float tq_out = 0.0f;
         if (fl & BF_STATIC) {
             st.po = st.p;
             vel = make_float2(0.0f, 0.0f);
+            if (GATHER) tq_out = k.src.torque[g_own];
         } else {
-            float2 acc = a.fresh ? k.s.acc[s] : make_float2(0.0f, 0.0f);
+            float2 acc = a.fresh ? k.src.acc[g_own] : make_float2(0.0f, 0.0f);
             acc.x = acc.x + a.gx * mgv.y;
             acc.y = acc.y + a.gy * mgv.y;
             const bool vr = (fl & BF_HAS_VREQ) != 0u;
-            const float2 vreq = vr ? k.s.vreq[s] : make_float2(0.0f, 0.0f);
-            const float tq = a.fresh ? k.s.torque[s] : 0.0f;
-            const float toi = (a.fresh || !(fl & BF_INERTIA_POS)) ? tq / k.s.inertia[s] : 0.0f;
+            const float2 vreq = vr ? k.src.vreq[g_own] : make_float2(0.0f, 0.0f);
+            const float tq = a.fresh ? k.src.torque[g_own] : 0.0f;
+            const float toi = (a.fresh || !(fl & BF_INERTIA_POS)) ? tq / k.src.inertia[g_own] : 0.0f;
             float ratio = a.ratio_rest;
             if ((int32_t)id == a.first_dyn) {
                 ratio = a.h / k.accum->old_dt;
@@ -287,42 +296,48 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             }
             verlet_dynamic(st, acc, toi, vr, vreq, a.h, ratio);
             vel = st.vel;
-            if (__float_as_uint(st.rot) != __float_as_uint(rw.x) || __float_as_uint(st.omega) != __float_as_uint(rw.y))
-                k.s.rw[s] = make_float2(st.rot, st.omega);
-            if (vr) k.s.bflags[s] = fl & ~BF_HAS_VREQ;
-            if (a.fresh) k.s.torque[s] = 0.0f;
+            if (vr) fl_out = fl & ~BF_HAS_VREQ;
         }
-        k.s.vel[s] = vel;
+        if (GATHER) {
+            // first substep after a rebuild: every record moves to its new slot
+            k.dst.rw[s] = make_float2(st.rot, st.omega);
+            k.dst.mg[s] = mgv;
+            k.dst.bflags[s] = fl_out;
+            k.dst.inertia[s] = k.src.inertia[g_own];
+            k.dst.torque[s] = tq_out;
+            k.dst.off[s] = k.src.off[g_own];
+            if (GROUPS) k.dst.groups[s] = k.src.groups[g_own];
+            if (fl_out & BF_HAS_VREQ) k.dst.vreq[s] = k.src.vreq[g_own];
+        } else {
+            if (__float_as_uint(st.rot) != __float_as_uint(rw.x) || __float_as_uint(st.omega) != __float_as_uint(rw.y))
+                k.dst.rw[s] = make_float2(st.rot, st.omega);
+            if (fl_out != fl) k.dst.bflags[s] = fl_out;
+            if (a.fresh && !(fl & BF_STATIC)) k.dst.torque[s] = 0.0f;
+        }
+        k.dst.vel[s] = vel;
         // collider transform from the pre-constraint position (physics.rs:360-366)
-        const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform_slow(st.rot, st.p, k.s.off[s]);
+        const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform_slow(st.rot, st.p, k.src.off[g_own]);
         const bool finite_x = finitef(x.x) && finitef(x.y);
         bool out = was_out;
-        if (!out && finite_x) {
+        if (a.collisions && !out && finite_x) {
             const float2 x0 = k.x0[s];
             const float ex = (x.x - x0.x) - a.drift_x, ey = (x.y - x0.y) - a.drift_y;
             out = !(sqrtf(ex * ex + ey * ey) <= a.skin);
             new_outliers += out ? 1u : 0u;
         }
         k.snap_out[s] = make_float4(x.x, x.y, own.z, __uint_as_float(id | (out ? kSnapOutlier : 0u)));
-        if (out && finite_x && a.collisions) {
-            const GridParams g = *k.grid;
+        if (out && finite_x) {
             // outlier table of the next substep
-            const uint32_t h = cell_hash(free_cell(x.x, g.ox, g.inv_cell), free_cell(x.y, g.oy, g.inv_cell), k.oh_mask);
+            const GridParams g = *k.grid;
+            const uint32_t h = cell_hash(free_cell(x.x, g.ox, g.inv_cell) >> kCoarseShift,
+                                         free_cell(x.y, g.oy, g.inv_cell) >> kCoarseShift, k.oh_mask);
             const uint32_t old = atomicExch(&k.oh_ins_head[h], s);
             k.oh_ins_next[s] = old;
-            // every body whose rebuild cell is within reach gets this slot on its list
-            grid_rows(g, k.cell_start, k.nb, x.x - a.drift_x, x.y - a.drift_y, own.z + a.rmax + a.skin,
-                      [&](uint32_t t0, uint32_t t1) {
-                          for (uint32_t t = t0; t < t1; ++t) {
-                              if (t == s) continue;
-                              const uint32_t c = atomicAdd(&k.xl_count_out[t], 1u);
-                              if (c < kExtra) k.xl_list_out[t * kExtra + c] = s;
-                          }
-                      });
+            atomicOr(&k.oh_ins_bits[h >> 5], 1u << (h & 31u));
         }
         la.point(x.x, x.y, x.x, x.y, 0.0f, 0.0f);
         if (CONSTRAINTS) apply_constraints(st.p, k.con_pos, k.con_r, k.nk);
-        k.s.pp[s] = make_float4(st.p.x, st.p.y, st.po.x, st.po.y);
+        k.dst.pp[s] = make_float4(st.p.x, st.p.y, st.po.x, st.po.y);
         la.contacts = count;
         la.push_outs = pushes;
         la.flags = flags;
@@ -339,68 +354,108 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
     }
 }
 
-struct SpherePermute {
-    SphereArrays src, dst;
-    const uint32_t* __restrict__ vals;
+// ---- rebuild: one-pass counting sort by cell ------------------------------
+
+// Cell key of every snapshot on the grid fitted to the bounding box
+// accumulated over the previous period, and its rank inside the cell.
+__global__ __launch_bounds__(kThreads) void k_sphere_count(const float4* __restrict__ snap, uint32_t nb,
+                                                           StepAccum* accum, uint32_t parity, float cell,
+                                                           uint32_t cells_cap, GridParams* grid_out,
+                                                           uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
+                                                           uint32_t* __restrict__ rank) {
+    const GridParams g = grid_from_bbox(accum->bbox[parity], cell, cells_cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *grid_out = g;
+        accum->bbox[parity ^ 1][0] = 0xFFFFFFFFu;
+        accum->bbox[parity ^ 1][1] = 0xFFFFFFFFu;
+        accum->bbox[parity ^ 1][2] = 0u;
+        accum->bbox[parity ^ 1][3] = 0u;
+        accum->dmax = 0u;
+        accum->n_outliers = 0u;
+    }
+    if (i < nb) {
+        const float4 sn = snap[i];
+        const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
+        const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
+        const uint32_t kk = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+        key[i] = kk;
+        rank[i] = atomicAdd(&cnt[kk], 1u);
+    }
+}
+
+struct SphereScatter {
+    const float4* __restrict__ snap;
+    float4* __restrict__ snap_sorted;
     float2* __restrict__ x0;
+    uint32_t* __restrict__ perm;
+    const uint32_t* __restrict__ key;
+    const uint32_t* __restrict__ rank;
+    const uint32_t* __restrict__ cell_start;
+    uint32_t* cnt;
+    uint32_t cells_cap;
     uint32_t* heads[3];
+    uint32_t* bits[3];
     uint32_t oh_size;
-    uint32_t* xl_count[2];
     uint32_t nb;
-    uint32_t carry_user;
 };
 
-// Gather every per-body record into cell order: dst[s] = src[vals[s]]; the
-// new snapshot is also the new x0, no body is an outlier, the outlier tables
-// and candidate lists are empty.
-__global__ __launch_bounds__(kThreads) void k_sphere_permute(SpherePermute k) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t i = s; i < 3u * k.oh_size; i += gridDim.x * blockDim.x) k.heads[i / k.oh_size][i % k.oh_size] = kNil;
-    if (s >= k.nb) return;
-    k.xl_count[0][s] = 0u;
-    k.xl_count[1][s] = 0u;
-    const uint32_t o = k.vals[s];
-    if (o >= k.nb) return;  // a permutation of [0, nb) by construction
-    k.dst.pp[s] = k.src.pp[o];
-    k.dst.vel[s] = k.src.vel[o];
-    k.dst.rw[s] = k.src.rw[o];
-    k.dst.mg[s] = k.src.mg[o];
-    const uint32_t fl = k.src.bflags[o];
-    k.dst.bflags[s] = fl;
-    k.dst.off[s] = k.src.off[o];
-    k.dst.groups[s] = k.src.groups[o];
-    k.dst.inertia[s] = k.src.inertia[o];
-    if (k.carry_user) {
-        k.dst.acc[s] = k.src.acc[o];
-        k.dst.torque[s] = k.src.torque[o];
+// Each snapshot to slot cell_start[key] + rank; the permutation records
+// where its body record is; x0 = the new snapshot; no body is an outlier;
+// counters and outlier tables are reset for the next period.
+__global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = i; c <= k.cells_cap; c += stride) k.cnt[c] = 0u;
+    for (uint32_t h = i; h < 3u * k.oh_size; h += stride) {
+        const uint32_t t = h / k.oh_size, e = h % k.oh_size;
+        k.heads[t][e] = kNil;
+        if ((e & 31u) == 0u) k.bits[t][e >> 5] = 0u;
     }
-    if (k.carry_user || (fl & BF_HAS_VREQ)) k.dst.vreq[s] = k.src.vreq[o];
-    float4 sn = k.src.snap[o];
+    if (i >= k.nb) return;
+    const uint32_t slot = k.cell_start[k.key[i]] + k.rank[i];
+    if (slot >= k.nb) return;  // consistent counts and scan never get here
+    float4 sn = k.snap[i];
     sn.w = __uint_as_float(__float_as_uint(sn.w) & ~kSnapOutlier);
-    k.dst.snap[s] = sn;
-    k.x0[s] = make_float2(sn.x, sn.y);
+    k.snap_sorted[slot] = sn;
+    k.x0[slot] = make_float2(sn.x, sn.y);
+    k.perm[slot] = i;
 }
 
 }  // namespace
 
-hipError_t launch_sphere_permute(DeviceWorld& d, bool carry_user_inputs) {
+hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     if (d.nb == 0) return hipSuccess;
-    SpherePermute k;
-    k.src = d.s;
-    k.dst = d.s_alt;
-    k.vals = d.vals;
-    k.x0 = d.x0;
-    for (int i = 0; i < 3; ++i) k.heads[i] = d.oh_head[i];
-    k.oh_size = d.oh_size;
-    k.xl_count[0] = d.xl_count[0];
-    k.xl_count[1] = d.xl_count[1];
-    k.nb = d.nb;
-    k.carry_user = carry_user_inputs ? 1u : 0u;
-    k_sphere_permute<<<div_up(d.nb, kThreads), kThreads, 0, d.stream>>>(k);
+    const uint32_t blocks = div_up(d.nb, kThreads);
+    k_sphere_count<<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.accum, parity, cell, d.cells_cap, d.grid,
+                                                      d.cell_count, d.keys, d.vals);
     hipError_t e = hipGetLastError();
-    SphereArrays t = d.s;
-    d.s = d.s_alt;
-    d.s_alt = t;
+    if (e != hipSuccess) return e;
+    e = exclusive_scan_u32(d.cell_count, d.cell_start, d.cells_cap + 1, d.sort, d.stream);
+    if (e != hipSuccess) return e;
+    SphereScatter k;
+    k.snap = d.s.snap;
+    k.snap_sorted = d.snap_next;
+    k.x0 = d.x0;
+    k.perm = d.perm;
+    k.key = d.keys;
+    k.rank = d.vals;
+    k.cell_start = d.cell_start;
+    k.cnt = d.cell_count;
+    k.cells_cap = d.cells_cap;
+    for (int t = 0; t < 3; ++t) {
+        k.heads[t] = d.oh_head[t];
+        k.bits[t] = d.oh_bits[t];
+    }
+    k.oh_size = d.oh_size;
+    k.nb = d.nb;
+    k_sphere_scatter<<<blocks, kThreads, 0, d.stream>>>(k);
+    e = hipGetLastError();
+    // the sorted snapshot becomes the current one; records follow on the next substep
+    float4* t = d.s.snap;
+    d.s.snap = d.snap_next;
+    d.snap_next = t;
+    d.gather_pending = true;
     return e;
 }
 
@@ -408,7 +463,11 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
                                  bool constraints) {
     if (d.nb == 0) return hipSuccess;
     SphereSubstep k;
-    k.s = d.s;
+    const bool gather = d.gather_pending;
+    k.src = d.s;
+    k.dst = gather ? d.s_alt : d.s;
+    k.dst.snap = d.s.snap;  // the snapshot is ping-ponged separately
+    k.perm = d.perm;
     k.snap_in = d.s.snap;
     k.snap_out = d.snap_next;
     k.x0 = d.x0;
@@ -417,14 +476,13 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     const uint32_t j = a.sub;
     k.oh_head = d.oh_head[j % 3];
     k.oh_next = d.oh_next[j % 3];
+    k.oh_bits = d.oh_bits[j % 3];
     k.oh_ins_head = d.oh_head[(j + 1) % 3];
     k.oh_ins_next = d.oh_next[(j + 1) % 3];
+    k.oh_ins_bits = d.oh_bits[(j + 1) % 3];
     k.oh_clr_head = d.oh_head[(j + 2) % 3];
+    k.oh_clr_bits = d.oh_bits[(j + 2) % 3];
     k.oh_mask = d.oh_size - 1;
-    k.xl_count_in = d.xl_count[j % 2];
-    k.xl_list_in = d.xl_list[j % 2];
-    k.xl_count_out = d.xl_count[(j + 1) % 2];
-    k.xl_list_out = d.xl_list[(j + 1) % 2];
     k.nb = d.nb;
     k.accum = d.accum;
     k.block_contacts = d.block_contacts;
@@ -432,17 +490,35 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.con_r = d.con_r;
     k.nk = d.nk;
     const uint32_t blocks = div_up(d.nb, kThreads);
-#define BG_SPHERE(G, S, Z, C) k_sphere_substep<G, S, Z, C><<<blocks, kThreads, 0, d.stream>>>(k, a)
+#define BG_SPHERE(G, S, Z, C, P) k_sphere_substep<G, S, Z, C, P><<<blocks, kThreads, 0, d.stream>>>(k, a)
+#define BG_SPHERE_P(G, S, Z, C)    \
+    do {                            \
+        if (gather)                 \
+            BG_SPHERE(G, S, Z, C, true); \
+        else                        \
+            BG_SPHERE(G, S, Z, C, false); \
+    } while (0)
     if (!groups && !sensors && zero_off && !constraints)
-        BG_SPHERE(false, false, true, false);
+        BG_SPHERE_P(false, false, true, false);
     else if (!groups && !sensors && !constraints)
-        BG_SPHERE(false, false, false, false);
+        BG_SPHERE_P(false, false, false, false);
     else if (!groups && !sensors)
-        BG_SPHERE(false, false, false, true);
+        BG_SPHERE_P(false, false, false, true);
     else
-        BG_SPHERE(true, true, false, true);
+        BG_SPHERE_P(true, true, false, true);
+#undef BG_SPHERE_P
 #undef BG_SPHERE
     hipError_t e = hipGetLastError();
+    if (gather) {
+        // records now live in the alternate buffers, in cell order
+        float4* snap_cur = d.s.snap;
+        SphereArrays t = d.s;
+        d.s = d.s_alt;
+        d.s_alt = t;
+        d.s.snap = snap_cur;
+        d.s_alt.snap = nullptr;
+        d.gather_pending = false;
+    }
     float4* t = d.s.snap;
     d.s.snap = d.snap_next;
     d.snap_next = t;

@@ -27,9 +27,9 @@ namespace bg {
 
 namespace {
 
-// Cell key of every snapshot (x at xy[i*stride], y next to it) on the grid
-// fitted to the bounding box accumulated since the previous rebuild.
-__global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float* __restrict__ xy, uint32_t stride, uint32_t nc,
+// Cell key of every collider snapshot on the grid fitted to the bounding box
+// accumulated since the previous rebuild; x0 keeps the snapshot.
+__global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float2* __restrict__ xy, uint32_t nc,
                                                            StepAccum* accum, uint32_t parity, float cell,
                                                            uint32_t cells_cap, GridParams* grid_out,
                                                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
@@ -46,12 +46,12 @@ __global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float* __restri
         accum->n_outliers = 0u;
     }
     if (i < nc) {
-        float2 x = make_float2(xy[(size_t)i * stride], xy[(size_t)i * stride + 1]);
+        const float2 x = xy[i];
         int32_t cx = cell_coord(x.x, g.ox, g.inv_cell, g.nx);
         int32_t cy = cell_coord(x.y, g.oy, g.inv_cell, g.ny);
         keys[i] = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
         vals[i] = i;
-        if (x0) x0[i] = x;
+        x0[i] = x;
     }
 }
 
@@ -366,13 +366,11 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
 
 // ============================================================ launchers
 
-hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity, bool sphere_path) {
+hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity) {
     uint32_t n = d.nc;
     uint32_t blocks = div_up(n > 0 ? n : 1, kThreads);
-    const float* xy = sphere_path ? reinterpret_cast<const float*>(d.s.snap) : reinterpret_cast<const float*>(d.c.abs);
-    k_rebuild_keys<<<blocks, kThreads, 0, d.stream>>>(xy, sphere_path ? 4u : 2u, n, d.accum, parity, cell,
-                                                      d.cells_cap, d.grid, d.keys, d.vals,
-                                                      sphere_path ? nullptr : d.x0);
+    k_rebuild_keys<<<blocks, kThreads, 0, d.stream>>>(d.c.abs, n, d.accum, parity, cell, d.cells_cap, d.grid, d.keys,
+                                                      d.vals, d.x0);
     return hipGetLastError();
 }
 

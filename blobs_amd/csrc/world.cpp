@@ -530,8 +530,8 @@ void World::free_arrays() {
         dfree(q->pp), dfree(q->snap), dfree(q->vel), dfree(q->rw), dfree(q->mg), dfree(q->bflags), dfree(q->off);
         dfree(q->groups), dfree(q->inertia), dfree(q->torque), dfree(q->acc), dfree(q->vreq);
     }
-    for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]);
-    for (int i = 0; i < 2; ++i) dfree(d.xl_count[i]), dfree(d.xl_list[i]);
+    for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]);
+    dfree(d.cell_count), dfree(d.perm);
     dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
@@ -824,12 +824,12 @@ int World::upload() {
         for (int i = 0; i < 3; ++i) {
             HIP_TRY(dmalloc(&d.oh_head[i], d.oh_size));
             HIP_TRY(dmalloc(&d.oh_next[i], nb));
+            HIP_TRY(dmalloc(&d.oh_bits[i], d.oh_size / 32u));
         }
-        for (int i = 0; i < 2; ++i) {
-            HIP_TRY(dmalloc(&d.xl_count[i], nb));
-            HIP_TRY(dmalloc(&d.xl_list[i], (size_t)nb * kExtra));
-            HIP_TRY(hipMemsetAsync(d.xl_count[i], 0, sizeof(uint32_t) * nb, s));
-        }
+        HIP_TRY(dmalloc(&d.cell_count, (size_t)d.cells_cap + 1));
+        HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), s));
+        HIP_TRY(dmalloc(&d.perm, nb));
+        d.gather_pending = false;
         std::vector<float4> pp(nb), snap(nb);
         std::vector<float2> rw(nb), mg(nb);
         for (uint32_t i = 0; i < nb; ++i) {
@@ -926,7 +926,7 @@ int World::upload() {
     d.n_block_slots = (std::max<uint32_t>(nb, 1) + 255) / 256;
     HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
     HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
-    HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, 1)));
+    HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, d.cells_cap + 1)));
     HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), d.stream));
     d.allocated = true;
 
@@ -1061,9 +1061,12 @@ int World::rebuild() {
     if (!isfinite(cell)) cell = 1.0f;
     const bool sphere = path_ == StepPath::Sphere;
     if (int st = stage_begin(BG_STAGE_REBUILD)) return st;
-    HIP_TRY(launch_rebuild_keys(d, cell, bbox_parity_, sphere));
-    HIP_TRY(launch_rebuild_sort_and_cells(d));
-    if (sphere) HIP_TRY(launch_sphere_permute(d, fresh_));
+    if (sphere) {
+        HIP_TRY(launch_sphere_rebuild(d, cell, bbox_parity_));
+    } else {
+        HIP_TRY(launch_rebuild_keys(d, cell, bbox_parity_));
+        HIP_TRY(launch_rebuild_sort_and_cells(d));
+    }
     if (int st = stage_end(BG_STAGE_REBUILD)) return st;
     bbox_parity_ ^= 1u;
     drift_x_ = drift_y_ = vdrift_x_ = vdrift_y_ = 0.0;

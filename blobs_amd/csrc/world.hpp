@@ -119,10 +119,10 @@ private:
     uint32_t steps_since_rebuild_ = 0;
     uint32_t bbox_parity_ = 0;
     float cell_scale_ = 1.0f;
-    float skin_scale_ = 0.125f;  // outlier skin as a fraction of the cell size
+    float skin_scale_ = 0.5f;  // outlier skin as a fraction of the cell size
     float skin_ = 0.0f;
     uint32_t sub_since_rebuild_ = 0;
-    uint32_t rebuild_interval_ = 4;
+    uint32_t rebuild_interval_ = 1;
     // world properties fixed at upload
     float rmax_ = 0.0f;
     int32_t first_dyn_ = -1;
