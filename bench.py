@@ -73,6 +73,22 @@ def cpu_baseline(scene, budget_s: float) -> dict:
     }
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "k_sphere_substep_plain_1M_traffic.json"
+
+
+def traffic_per_launch(n_bodies: int):
+    """HBM bytes per substep launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, each calibrated by a 256 MiB copy kernel), from the committed
+    profile summary of the same kernel and workload; None if absent."""
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if rec.get("bodies") != n_bodies:
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +100,8 @@ def main() -> int:
     ap.add_argument("--rebuild-interval", type=int, default=None)
     ap.add_argument("--cell-scale", type=float, default=None)
     ap.add_argument("--skin", type=float, default=None)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 256 MiB device copy (hardware byte-counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -117,6 +135,13 @@ def main() -> int:
     n_sub, ms_sub = p.stage_stats(0)
     n_reb, ms_reb = p.stage_stats(1)
     p.set_profiling(False)
+
+    if args.copy_probe:
+        import ctypes as C
+        p._lib.bgx_copy_probe.argtypes = [C.c_uint64, C.c_uint32]
+        if p._lib.bgx_copy_probe(256 << 20, 3) != 0:
+            log("copy probe failed")
+            return 1
 
     pos = p.bodies()["position"]
     if not np.isfinite(pos).all():
@@ -160,7 +185,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_per_launch(sc.n_bodies),
             "bytes_per_launch": BYTES_PER_BODY_SUBSTEP * sc.n_bodies,
             "avg_launch_us": avg_sub_ms * 1e3,
             "rebuild_avg_us": ms_reb / max(n_reb, 1) * 1e3,

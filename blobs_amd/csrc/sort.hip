@@ -259,6 +259,29 @@ done:
     return (int)e;
 }
 
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Copies `bytes` (a multiple of 16) between two device buffers with a plain
+// grid-stride kernel, `repeats` times: the known-size reference that hardware
+// byte counters are calibrated against.
+extern "C" int bgx_copy_probe(uint64_t bytes, uint32_t repeats) {
+    if (bytes == 0 || bytes % 16 != 0) return (int)hipErrorInvalidValue;
+    uint4 *a = nullptr, *b = nullptr;
+    hipError_t e = hipSuccess;
+    const size_t n = bytes / 16;
+    if ((e = hipMalloc(&a, bytes)) || (e = hipMalloc(&b, bytes))) goto done;
+    if ((e = hipMemset(a, 1, bytes))) goto done;
+    for (uint32_t r = 0; r < repeats; ++r) k_copy_probe<<<2048, 256>>>(a, b, n);
+    if ((e = hipGetLastError())) goto done;
+    e = hipDeviceSynchronize();
+done:
+    (void)hipFree(a), (void)hipFree(b);
+    return (int)e;
+}
+
 extern "C" int bgx_exclusive_scan_u32(const uint32_t* in, uint32_t n, uint32_t* out) {
     uint32_t* d = nullptr;
     SortTemp t;
