@@ -168,9 +168,25 @@ SIGNATURES = {
     "bg_step_path": [WORLD, u32p],
     "bg_set_profiling": [WORLD, C.c_int],
     "bg_stage_stats": [WORLD, C.c_uint32, u64p, C.POINTER(C.c_double)],
+    "bg_batch_run": [WORLD, C.c_double, C.c_uint32, C.c_uint32],
+    "bg_batch_create": [C.POINTER(WORLD), C.c_uint32, C.POINTER(WORLD)],
+    "bg_batch_writeback": [WORLD, C.POINTER(WORLD), C.c_uint32],
+    "bg_batch_size": [WORLD, u32p],
+    "bg_autotune": [WORLD, C.c_double, C.c_uint32, f32p, f32p, u32p],
 }
 
-# entry points added by the batched and differentiable forms, bound when present
+ROLLOUT = C.c_void_p
+SIGNATURES.update({
+    "bg_rollout_create": [C.POINTER(WORLD), C.c_uint32, f32p, C.c_uint32, C.POINTER(ROLLOUT)],
+    "bg_rollout_destroy": [ROLLOUT],
+    "bg_rollout_sizes": [ROLLOUT, u32p, u32p, u32p],
+    "bg_rollout_set_params": [ROLLOUT, f32p, f32p],
+    "bg_rollout_forward": [ROLLOUT, C.c_double, C.c_uint32, f32p],
+    "bg_rollout_backward": [ROLLOUT, f32p, f32p, f32p],
+    "bg_rollout_final_positions": [ROLLOUT, f32p],
+})
+
+# entry points bound only when present in the loaded library
 EXTRA_SIGNATURES: dict[str, list] = {}
 
 _lib: C.CDLL | None = None

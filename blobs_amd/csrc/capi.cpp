@@ -5,13 +5,19 @@
 #include <exception>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/blobs_gpu.h"
+#include "diff.hpp"
 #include "world.hpp"
 
 struct bg_world {
     bg::World impl;
     bg_world(float gx, float gy, bool sh) : impl(gx, gy, sh) {}
+};
+
+struct bg_rollout {
+    bg::Rollout impl;
 };
 
 namespace {
@@ -33,6 +39,22 @@ int guarded(bg_world* w, F&& f) {
         return BG_ERR_CAPACITY;
     } catch (const std::exception& e) {
         w->impl.last_error = g_last_error = e.what();
+        return BG_ERR_PANIC;
+    }
+}
+
+template <class F>
+int rguard(bg_rollout* r, F&& f) {
+    if (!r) {
+        g_last_error = "null rollout";
+        return BG_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        int st = f(r->impl);
+        if (st != BG_OK) g_last_error = r->impl.last_error;
+        return st;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
         return BG_ERR_PANIC;
     }
 }
@@ -134,6 +156,57 @@ int bg_synchronize(bg_world* w) {
     return guarded(w, [&](bg::World& x) { return x.synchronize(); });
 }
 
+int bg_batch_run(bg_world* w, double delta, uint32_t chunk_steps, uint32_t n_chunks) {
+    return guarded(w, [&](bg::World& x) { return x.step_chunks(delta, chunk_steps, n_chunks); });
+}
+
+int bg_batch_create(bg_world* const* worlds, uint32_t n_worlds, bg_world** out) {
+    if (!out || !worlds || n_worlds == 0 || !worlds[0]) {
+        g_last_error = "bg_batch_create: worlds, n_worlds > 0 and out are required";
+        return BG_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    bg_params p;
+    int st = bg_get_params(worlds[0], &p);
+    if (st != BG_OK) return st;
+    bg_world* b = nullptr;
+    try {
+        b = new bg_world(p.gravity[0], p.gravity[1], p.use_spatial_hash != 0);
+    } catch (...) {
+        g_last_error = "allocation failed";
+        return BG_ERR_CAPACITY;
+    }
+    std::vector<bg::World*> src(n_worlds);
+    for (uint32_t i = 0; i < n_worlds; ++i) src[i] = worlds[i] ? &worlds[i]->impl : nullptr;
+    st = guarded(b, [&](bg::World& x) { return x.init_batch(src.data(), n_worlds); });
+    if (st != BG_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return BG_OK;
+}
+
+int bg_batch_writeback(bg_world* batch, bg_world* const* worlds, uint32_t n_worlds) {
+    if (!worlds) return BG_ERR_INVALID_ARGUMENT;
+    std::vector<bg::World*> dst(n_worlds);
+    for (uint32_t i = 0; i < n_worlds; ++i) dst[i] = worlds[i] ? &worlds[i]->impl : nullptr;
+    return guarded(batch, [&](bg::World& x) { return x.batch_writeback(dst.data(), n_worlds); });
+}
+
+int bg_batch_size(bg_world* w, uint32_t* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) {
+        *out = x.batch_size();
+        return BG_OK;
+    });
+}
+
+int bg_autotune(bg_world* w, double delta, uint32_t steps_per_trial, float* cell_scale, float* skin,
+                uint32_t* rebuild_interval) {
+    return guarded(w, [&](bg::World& x) { return x.autotune(delta, steps_per_trial, cell_scale, skin, rebuild_interval); });
+}
+
 int bg_rbd_count(bg_world* w, uint32_t* out) {
     if (!out) return BG_ERR_INVALID_ARGUMENT;
     return guarded(w, [&](bg::World& x) {
@@ -208,6 +281,71 @@ int bg_step_path(bg_world* w, uint32_t* out) {
 // step record {n_outliers, dmax bits, push_outs, error_flags}.
 int bgx_world_accum(bg_world* w, uint32_t* out4) {
     return guarded(w, [&](bg::World& x) { return x.debug_accum(out4); });
+}
+
+int bg_rollout_create(bg_world* const* worlds, uint32_t n_worlds, const float* targets_xy, uint32_t checkpoint_every,
+                      bg_rollout** out) {
+    if (!out || !worlds || !targets_xy || n_worlds == 0) {
+        g_last_error = "bg_rollout_create: worlds, n_worlds > 0, targets and out are required";
+        return BG_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    bg_rollout* r = nullptr;
+    try {
+        r = new bg_rollout();
+        std::vector<bg::World*> src(n_worlds);
+        for (uint32_t i = 0; i < n_worlds; ++i) src[i] = worlds[i] ? &worlds[i]->impl : nullptr;
+        int st = r->impl.create(src.data(), n_worlds, targets_xy, checkpoint_every);
+        if (st != BG_OK) {
+            g_last_error = r->impl.last_error;
+            delete r;
+            return st;
+        }
+    } catch (const std::bad_alloc&) {
+        delete r;
+        g_last_error = "host out of memory";
+        return BG_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        delete r;
+        g_last_error = e.what();
+        return BG_ERR_PANIC;
+    }
+    *out = r;
+    return BG_OK;
+}
+
+int bg_rollout_destroy(bg_rollout* r) {
+    delete r;
+    return BG_OK;
+}
+
+int bg_rollout_sizes(bg_rollout* r, uint32_t* n_worlds, uint32_t* n_bodies, uint32_t* n_springs) {
+    return rguard(r, [&](bg::Rollout& x) {
+        if (n_worlds) *n_worlds = x.n_worlds();
+        if (n_bodies) *n_bodies = x.n_bodies();
+        if (n_springs) *n_springs = x.n_springs();
+        return BG_OK;
+    });
+}
+
+int bg_rollout_set_params(bg_rollout* r, const float* velocity_request, const float* stiffness) {
+    return rguard(r, [&](bg::Rollout& x) { return x.set_params(velocity_request, stiffness); });
+}
+
+int bg_rollout_forward(bg_rollout* r, double delta, uint32_t n_steps, float* loss_per_world) {
+    return rguard(r, [&](bg::Rollout& x) { return x.forward(delta, n_steps, loss_per_world); });
+}
+
+int bg_rollout_backward(bg_rollout* r, const float* dloss_dfinal_position, float* grad_velocity_request,
+                        float* grad_stiffness) {
+    return rguard(r, [&](bg::Rollout& x) {
+        return x.backward(dloss_dfinal_position, grad_velocity_request, grad_stiffness);
+    });
+}
+
+int bg_rollout_final_positions(bg_rollout* r, float* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return rguard(r, [&](bg::Rollout& x) { return x.final_positions(out); });
 }
 
 int bg_set_profiling(bg_world* w, int enable) {

@@ -145,6 +145,7 @@ enum : uint32_t {
     BF_HAS_VREQ = 1u << 1,
     BF_INERTIA_POS = 1u << 2,  // inertia is positive and finite
     BF_SENSOR = 1u << 3,       // sphere path: the body's collider is a sensor
+    BF_FIRST_DYN = 1u << 4,    // sphere path: first non-static body of its world (Q1, physics.rs:338-340)
 };
 
 struct GridParams {

@@ -76,6 +76,16 @@ struct DeviceWorld {
     uint32_t *cell_count = nullptr;
     uint32_t *perm = nullptr;
     bool gather_pending = false;
+    // worlds of the sphere path (one for a plain world, many for a batch):
+    // bodies of world w keep the slot range [body_start[w], body_start[w+1])
+    // through every sort because their cell keys start at cell_base[w]
+    uint32_t n_worlds = 0;
+    uint32_t *wid = nullptr;        // world of each slot (batch only)
+    uint32_t *cell_base = nullptr;  // [n_worlds + 1]
+    GridParams *grids = nullptr;    // [n_worlds]
+    uint32_t *wbbox = nullptr;      // [2][n_worlds][4] f2ord(min x, min y, max x, max y) per parity
+    float *wold_dt = nullptr;       // [n_worlds] Physics::old_dt of each world (Q1)
+    uint32_t total_cells = 0;       // cell_base[n_worlds]
     // springs
     int32_t *sp_a = nullptr, *sp_b = nullptr;
     float *sp_rest = nullptr, *sp_k = nullptr, *sp_c = nullptr;
@@ -129,5 +139,7 @@ hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t
 hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity);
 hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool groups, bool sensors, bool zero_off,
                                  bool constraints);
+// fills n copies of v (graph-safe replacement of hipMemsetAsync)
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 }  // namespace bg

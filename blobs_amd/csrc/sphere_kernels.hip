@@ -1,27 +1,31 @@
 // sphere_kernels.hip -- the fused substep for worlds whose every body owns
 // exactly one collider and that have no springs or joints (the plain large
-// scene): gravity, contacts, Verlet, collider transform and circle
-// constraints of physics.rs:401-421 in ONE kernel per substep.
+// scene, and batches of such worlds): gravity, contacts, Verlet, collider
+// transform and circle constraints of physics.rs:401-421 in ONE kernel per
+// substep, over every world of a batch at once.
 //
 // Layout: bodies sit in broadphase-cell order in packed 16-byte records; the
 // collider snapshot the contact pass reads (physics.rs:264, written by the
 // previous substep's update_objects at :360-366) is ping-ponged so a
-// substep can read every snapshot while writing its own.
+// substep can read every snapshot while writing its own.  World w owns the
+// cell keys [cell_base[w], cell_base[w+1]), so sorting by key keeps its
+// bodies in the fixed slot range they were uploaded to; a query never
+// leaves its world's cells, so pairs never cross worlds.
 //
-// Broadphase.  Every step starts with a one-pass counting sort of the
-// snapshots by grid cell (cell-hash keys, atomic ranks, a decoupled
-// look-back scan for the cell table, a scatter of the snapshot records and
-// of the permutation).  The first substep after it gathers the body records
-// through the permutation and writes them back in cell order, so the
-// reordering costs no separate pass.  Between rebuilds the sorted grid keeps
-// the rebuild snapshots (x0); a body whose snapshot moves more than `skin`
-// from x0 (after removing the common gravity drift) becomes an outlier and
-// is, from then on, re-inserted every substep into a coarse hashed cell
-// table with an occupancy bitmap.  A query scans the sorted grid with radius
-// r + r_max + skin (skipping outliers) and the outlier table with radius
-// r + r_max, so every pair of the reference predicate is visited exactly
-// once from each side, whatever the bodies do; skin, cell size and rebuild
-// interval change only the speed.
+// Broadphase.  A rebuild fits each world's grid to the bounding box of its
+// snapshots over the previous period, counts bodies per cell with atomic
+// ranks, scans the counts into the cell table and scatters the snapshot
+// records and the permutation.  The first substep after it gathers the body
+// records through the permutation and writes them back in cell order, so
+// the reordering costs no separate pass.  Between rebuilds the sorted grid
+// keeps the rebuild snapshots (x0); a body whose snapshot moves more than
+// `skin` from x0 (after removing the common gravity drift) becomes an
+// outlier and is, from then on, re-inserted every substep into a coarse
+// hashed cell table with an occupancy bitmap.  A query scans the sorted grid
+// with radius r + r_max + skin (skipping outliers) and the outlier table
+// with radius r + r_max, so every pair of the reference predicate is
+// visited exactly once from each side, whatever the bodies do; skin, cell
+// size and rebuild interval change only the speed.
 #include "common.hpp"
 #include "device_state.hpp"
 #include "step_common.hpp"
@@ -40,7 +44,11 @@ struct SphereSubstep {
     float4* __restrict__ snap_out;
     const float2* __restrict__ x0;
     const uint32_t* __restrict__ cell_start;
-    const GridParams* __restrict__ grid;
+    const uint32_t* __restrict__ wid;
+    const uint32_t* __restrict__ cell_base;
+    const GridParams* __restrict__ grids;
+    uint32_t* wbbox;  // this substep's parity: [n_worlds][4]
+    float* wold_dt;
     const uint32_t* __restrict__ oh_head;  // outlier table read this substep
     const uint32_t* __restrict__ oh_next;
     const uint32_t* __restrict__ oh_bits;
@@ -64,8 +72,8 @@ __device__ __forceinline__ int32_t free_cell(float x, float o, float inv) {
     return (int32_t)f;
 }
 
-__device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, uint32_t mask) {
-    return (((uint32_t)cx * 0x9E3779B1u) ^ ((uint32_t)cy * 0x85EBCA77u)) & mask;
+__device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, uint32_t w, uint32_t mask) {
+    return (((uint32_t)cx * 0x9E3779B1u) ^ ((uint32_t)cy * 0x85EBCA77u) ^ (w * 0xC2B2AE3Du)) & mask;
 }
 
 // Rows of the sorted grid overlapping the square of half-width Q around
@@ -96,7 +104,29 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
     }
 }
 
-template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER>
+// Bounding box of this substep's snapshots per world: one conditional atomic
+// set per wave when the wave lies in one world, per lane otherwise.
+__device__ __forceinline__ void flush_world_bbox(const LocalAccum& la, uint32_t w, uint32_t* wbbox) {
+    const uint32_t w0 = __shfl(w, 0, BG_WAVE);
+    const int lane = threadIdx.x & (BG_WAVE - 1);
+    uint32_t v0 = f2ord(la.minx), v1 = f2ord(la.miny), v2 = f2ord(la.maxx), v3 = f2ord(la.maxy);
+    if (__all(w == w0)) {
+        v0 = wave_min_u32(v0);
+        v1 = wave_min_u32(v1);
+        v2 = wave_max_u32(v2);
+        v3 = wave_max_u32(v3);
+        if (lane != 0 || v0 > v2) return;  // nothing finite in the wave
+    } else if (v0 > v2) {
+        return;
+    }
+    uint32_t* b = wbbox + (size_t)w * 4;
+    atomic_min_if_less(&b[0], v0);
+    atomic_min_if_less(&b[1], v1);
+    atomic_max_if_greater(&b[2], v2);
+    atomic_max_if_greater(&b[3], v3);
+}
+
+template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER, bool BATCHED>
 __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
     LocalAccum la;
     la.init();
@@ -107,6 +137,8 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         k.oh_clr_head[i] = kNil;
         if ((i & 31u) == 0u) k.oh_clr_bits[i >> 5] = 0u;
     }
+    // world of this slot (lanes past the end take the last body's world)
+    const uint32_t w = BATCHED ? k.wid[s < k.nb ? s : k.nb - 1] : 0u;
     uint32_t new_outliers = 0;
     if (s < k.nb) {
         const uint32_t g_own = GATHER ? k.perm[s] : s;
@@ -125,7 +157,8 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
             const float ra = own.z;
-            const GridParams g = *k.grid;
+            const GridParams g = k.grids[w];
+            const uint32_t* __restrict__ cs = k.cell_start + k.cell_base[w];
             uint2 ga = make_uint2(0, 0);
             if (GROUPS) ga = k.src.groups[g_own];
             const bool sen_a = SENSORS ? (fl & BF_SENSOR) != 0u : false;
@@ -149,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             };
             auto enumerate = [&](auto&& sink) {
                 // sorted grid: four candidates per load batch (clamped indices, no branches)
-                grid_rows(g, k.cell_start, k.nb, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, ra + a.rmax + a.skin,
+                grid_rows(g, cs, k.nb, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, ra + a.rmax + a.skin,
                           [&](uint32_t t0, uint32_t t1) {
                               for (uint32_t t = t0; t < t1; t += 4) {
                                   float4 sb[4];
@@ -164,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                               }
                           });
                 if (nout == 0u) return;
-                // outlier table: coarse cells around the current snapshot
+                // outlier table: coarse cells of this world around the current snapshot
                 float R = ra + a.rmax;
                 R = R + (1e-5f * (fabsf(xa.x) + fabsf(xa.y) + R) + 1e-6f);
                 const int32_t cx0 = free_cell(xa.x - R, g.ox, g.inv_cell) >> kCoarseShift;
@@ -174,15 +207,16 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                 if (!(R < 1e30f) || cx1 - cx0 > 64 || cy1 - cy0 > 64) {
                     for (uint32_t h = 0; h <= k.oh_mask; ++h)  // huge radius: every bucket (superset)
                         for (uint32_t t = k.oh_head[h]; t != kNil; t = k.oh_next[t])
-                            if (t != s) test(t, k.snap_in[t], sink);
+                            if (t != s && (!BATCHED || k.wid[t] == w)) test(t, k.snap_in[t], sink);
                     return;
                 }
                 for (int32_t cy = cy0; cy <= cy1; ++cy)
                     for (int32_t cx = cx0; cx <= cx1; ++cx) {
-                        const uint32_t h = cell_hash(cx, cy, k.oh_mask);
+                        const uint32_t h = cell_hash(cx, cy, w, k.oh_mask);
                         if (!((k.oh_bits[h >> 5] >> (h & 31u)) & 1u)) continue;
                         for (uint32_t t = k.oh_head[h]; t != kNil; t = k.oh_next[t]) {
                             if (t == s) continue;
+                            if (BATCHED && k.wid[t] != w) continue;
                             const float4 sb = k.snap_in[t];
                             // each outlier lives in one coarse cell: drop hash collisions
                             if ((free_cell(sb.x, g.ox, g.inv_cell) >> kCoarseShift) != cx ||
@@ -290,9 +324,9 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             const float tq = a.fresh ? k.src.torque[g_own] : 0.0f;
             const float toi = (a.fresh || !(fl & BF_INERTIA_POS)) ? tq / k.src.inertia[g_own] : 0.0f;
             float ratio = a.ratio_rest;
-            if ((int32_t)id == a.first_dyn) {
-                ratio = a.h / k.accum->old_dt;
-                k.accum->old_dt = a.h;
+            if (fl & BF_FIRST_DYN) {  // Q1: this world's old_dt
+                ratio = a.h / k.wold_dt[w];
+                k.wold_dt[w] = a.h;
             }
             verlet_dynamic(st, acc, toi, vr, vreq, a.h, ratio);
             vel = st.vel;
@@ -328,9 +362,9 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         k.snap_out[s] = make_float4(x.x, x.y, own.z, __uint_as_float(id | (out ? kSnapOutlier : 0u)));
         if (out && finite_x) {
             // outlier table of the next substep
-            const GridParams g = *k.grid;
+            const GridParams g = k.grids[w];
             const uint32_t h = cell_hash(free_cell(x.x, g.ox, g.inv_cell) >> kCoarseShift,
-                                         free_cell(x.y, g.oy, g.inv_cell) >> kCoarseShift, k.oh_mask);
+                                         free_cell(x.y, g.oy, g.inv_cell) >> kCoarseShift, w, k.oh_mask);
             const uint32_t old = atomicExch(&k.oh_ins_head[h], s);
             k.oh_ins_next[s] = old;
             atomicOr(&k.oh_ins_bits[h >> 5], 1u << (h & 31u));
@@ -342,46 +376,58 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         la.push_outs = pushes;
         la.flags = flags;
     }
+    flush_world_bbox(la, w, k.wbbox);
     // outlier count: one atomic per block that produced any
     const uint32_t no = wave_sum_u32(new_outliers);
     __shared__ uint32_t s_no[kThreads / BG_WAVE];
     if ((threadIdx.x & (BG_WAVE - 1)) == 0) s_no[threadIdx.x / BG_WAVE] = no;
-    flush_accum(la, k.accum, k.block_contacts, a.parity, true);
+    flush_accum(la, k.accum, k.block_contacts, a.parity, false);
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int w = 0; w < kThreads / BG_WAVE; ++w) tot += s_no[w];
+        for (int i = 0; i < kThreads / BG_WAVE; ++i) tot += s_no[i];
         if (tot) atomicAdd(&k.accum->n_outliers, tot);
     }
 }
 
 // ---- rebuild: one-pass counting sort by cell ------------------------------
 
-// Cell key of every snapshot on the grid fitted to the bounding box
-// accumulated over the previous period, and its rank inside the cell.
-__global__ __launch_bounds__(kThreads) void k_sphere_count(const float4* __restrict__ snap, uint32_t nb,
-                                                           StepAccum* accum, uint32_t parity, float cell,
-                                                           uint32_t cells_cap, GridParams* grid_out,
-                                                           uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
-                                                           uint32_t* __restrict__ rank) {
-    const GridParams g = grid_from_bbox(accum->bbox[parity], cell, cells_cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        *grid_out = g;
-        accum->bbox[parity ^ 1][0] = 0xFFFFFFFFu;
-        accum->bbox[parity ^ 1][1] = 0xFFFFFFFFu;
-        accum->bbox[parity ^ 1][2] = 0u;
-        accum->bbox[parity ^ 1][3] = 0u;
+// Grid of every world, fitted to the bounding box of its snapshots over the
+// period that ends here; the other parity's boxes restart empty.
+__global__ __launch_bounds__(kThreads) void k_world_grids(uint32_t n_worlds, uint32_t* wbbox, uint32_t parity,
+                                                          float cell, const uint32_t* __restrict__ cell_base,
+                                                          GridParams* grids, StepAccum* accum) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w == 0) {
         accum->dmax = 0u;
         accum->n_outliers = 0u;
     }
-    if (i < nb) {
-        const float4 sn = snap[i];
-        const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
-        const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
-        const uint32_t kk = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
-        key[i] = kk;
-        rank[i] = atomicAdd(&cnt[kk], 1u);
-    }
+    if (w >= n_worlds) return;
+    grids[w] = grid_from_bbox(wbbox + ((size_t)parity * n_worlds + w) * 4, cell, cell_base[w + 1] - cell_base[w]);
+    uint32_t* nb = wbbox + ((size_t)(parity ^ 1u) * n_worlds + w) * 4;
+    nb[0] = 0xFFFFFFFFu;
+    nb[1] = 0xFFFFFFFFu;
+    nb[2] = 0u;
+    nb[3] = 0u;
+}
+
+// Cell key of every snapshot and its rank inside the cell.
+template <bool BATCHED>
+__global__ __launch_bounds__(kThreads) void k_sphere_count(const float4* __restrict__ snap, uint32_t nb,
+                                                           const uint32_t* __restrict__ wid,
+                                                           const GridParams* __restrict__ grids,
+                                                           const uint32_t* __restrict__ cell_base,
+                                                           uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
+                                                           uint32_t* __restrict__ rank) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const uint32_t w = BATCHED ? wid[i] : 0u;
+    const GridParams g = grids[w];
+    const float4 sn = snap[i];
+    const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
+    const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
+    const uint32_t kk = cell_base[w] + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+    key[i] = kk;
+    rank[i] = atomicAdd(&cnt[kk], 1u);
 }
 
 struct SphereScatter {
@@ -393,7 +439,7 @@ struct SphereScatter {
     const uint32_t* __restrict__ rank;
     const uint32_t* __restrict__ cell_start;
     uint32_t* cnt;
-    uint32_t cells_cap;
+    uint32_t total_cells;
     uint32_t* heads[3];
     uint32_t* bits[3];
     uint32_t oh_size;
@@ -406,7 +452,7 @@ struct SphereScatter {
 __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = i; c <= k.cells_cap; c += stride) k.cnt[c] = 0u;
+    for (uint32_t c = i; c <= k.total_cells; c += stride) k.cnt[c] = 0u;
     for (uint32_t h = i; h < 3u * k.oh_size; h += stride) {
         const uint32_t t = h / k.oh_size, e = h % k.oh_size;
         k.heads[t][e] = kNil;
@@ -422,16 +468,34 @@ __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
     k.perm[slot] = i;
 }
 
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 }  // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_fill_u32<<<(uint32_t)blocks, 256, 0, s>>>(p, v, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     if (d.nb == 0) return hipSuccess;
     const uint32_t blocks = div_up(d.nb, kThreads);
-    k_sphere_count<<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.accum, parity, cell, d.cells_cap, d.grid,
-                                                      d.cell_count, d.keys, d.vals);
+    k_world_grids<<<div_up(d.n_worlds, kThreads), kThreads, 0, d.stream>>>(d.n_worlds, d.wbbox, parity, cell,
+                                                                           d.cell_base, d.grids, d.accum);
+    if (d.n_worlds > 1)
+        k_sphere_count<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.wid, d.grids, d.cell_base,
+                                                                d.cell_count, d.keys, d.vals);
+    else
+        k_sphere_count<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.wid, d.grids, d.cell_base,
+                                                                 d.cell_count, d.keys, d.vals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = exclusive_scan_u32(d.cell_count, d.cell_start, d.cells_cap + 1, d.sort, d.stream);
+    e = exclusive_scan_u32(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream);
     if (e != hipSuccess) return e;
     SphereScatter k;
     k.snap = d.s.snap;
@@ -442,7 +506,7 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     k.rank = d.vals;
     k.cell_start = d.cell_start;
     k.cnt = d.cell_count;
-    k.cells_cap = d.cells_cap;
+    k.total_cells = d.total_cells;
     for (int t = 0; t < 3; ++t) {
         k.heads[t] = d.oh_head[t];
         k.bits[t] = d.oh_bits[t];
@@ -464,6 +528,7 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     if (d.nb == 0) return hipSuccess;
     SphereSubstep k;
     const bool gather = d.gather_pending;
+    const bool batched = d.n_worlds > 1;
     k.src = d.s;
     k.dst = gather ? d.s_alt : d.s;
     k.dst.snap = d.s.snap;  // the snapshot is ping-ponged separately
@@ -472,7 +537,11 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.snap_out = d.snap_next;
     k.x0 = d.x0;
     k.cell_start = d.cell_start;
-    k.grid = d.grid;
+    k.wid = d.wid;
+    k.cell_base = d.cell_base;
+    k.grids = d.grids;
+    k.wbbox = d.wbbox + (size_t)a.parity * d.n_worlds * 4;
+    k.wold_dt = d.wold_dt;
     const uint32_t j = a.sub;
     k.oh_head = d.oh_head[j % 3];
     k.oh_next = d.oh_next[j % 3];
@@ -490,23 +559,27 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.con_r = d.con_r;
     k.nk = d.nk;
     const uint32_t blocks = div_up(d.nb, kThreads);
-#define BG_SPHERE(G, S, Z, C, P) k_sphere_substep<G, S, Z, C, P><<<blocks, kThreads, 0, d.stream>>>(k, a)
-#define BG_SPHERE_P(G, S, Z, C)    \
-    do {                            \
-        if (gather)                 \
-            BG_SPHERE(G, S, Z, C, true); \
-        else                        \
-            BG_SPHERE(G, S, Z, C, false); \
+#define BG_SPHERE(G, S, Z, C, P, B) k_sphere_substep<G, S, Z, C, P, B><<<blocks, kThreads, 0, d.stream>>>(k, a)
+#define BG_SPHERE_PB(G, S, Z, C)                 \
+    do {                                         \
+        if (gather && batched)                   \
+            BG_SPHERE(G, S, Z, C, true, true);   \
+        else if (gather)                         \
+            BG_SPHERE(G, S, Z, C, true, false);  \
+        else if (batched)                        \
+            BG_SPHERE(G, S, Z, C, false, true);  \
+        else                                     \
+            BG_SPHERE(G, S, Z, C, false, false); \
     } while (0)
     if (!groups && !sensors && zero_off && !constraints)
-        BG_SPHERE_P(false, false, true, false);
+        BG_SPHERE_PB(false, false, true, false);
     else if (!groups && !sensors && !constraints)
-        BG_SPHERE_P(false, false, false, false);
+        BG_SPHERE_PB(false, false, false, false);
     else if (!groups && !sensors)
-        BG_SPHERE_P(false, false, false, true);
+        BG_SPHERE_PB(false, false, false, true);
     else
-        BG_SPHERE_P(true, true, false, true);
-#undef BG_SPHERE_P
+        BG_SPHERE_PB(true, true, false, true);
+#undef BG_SPHERE_PB
 #undef BG_SPHERE
     hipError_t e = hipGetLastError();
     if (gather) {

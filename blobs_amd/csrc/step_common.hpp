@@ -249,7 +249,7 @@ __device__ __forceinline__ GridParams grid_from_bbox(const uint32_t* bb, float c
     int32_t nx, ny;
     for (;;) {
         float fx = floorf(ex / cell) + 1.0f, fy = floorf(ey / cell) + 1.0f;
-        if (fx * fy <= (float)cells_cap && fx <= 65536.0f && fy <= 65536.0f) {
+        if (fx <= 65536.0f && fy <= 65536.0f && (uint64_t)fx * (uint64_t)fy <= (uint64_t)cells_cap) {
             nx = (int32_t)fx;
             ny = (int32_t)fy;
             break;

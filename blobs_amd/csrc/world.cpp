@@ -532,6 +532,7 @@ void World::free_arrays() {
     }
     for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]);
     dfree(d.cell_count), dfree(d.perm);
+    dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt);
     dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
@@ -802,7 +803,66 @@ int World::upload() {
     d.key_bits = 0;
     while ((1u << d.key_bits) < d.cells_cap) ++d.key_bits;
     hipStream_t s = d.stream;
+    uint32_t cell_rows = d.cells_cap;  // entries of the cell table (+1)
+    if (!batch_start_.empty() && (!sphere || batch_start_.back() != nb)) {
+        structure_error_ = true;
+        structure_msg_ = "a batch holds single-sphere worlds only (one collider per body, no springs or joints)";
+    }
     if (sphere) {
+        // worlds: one for a plain world; a batch keeps its source worlds' body ranges
+        std::vector<uint32_t> wstart = batch_start_.empty() ? std::vector<uint32_t>{0u, nb} : batch_start_;
+        const uint32_t W = (uint32_t)wstart.size() - 1;
+        std::vector<uint32_t> cell_base(W + 1, 0), wid;
+        std::vector<float> wold(W, params_.old_dt);
+        if (!batch_old_dt_.empty()) wold = batch_old_dt_;
+        std::vector<uint32_t> wbbox(8u * W, 0u);
+        if (W > 1) wid.resize(nb);
+        for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t n_w = wstart[w + 1] - wstart[w];
+            const uint64_t cap = std::max<uint64_t>(4ull * n_w, 1024ull);
+            const uint64_t base = (uint64_t)cell_base[w] + ((cap + 63ull) & ~63ull);
+            if (base >= 0xFFFFFFF0ull) return fail(BG_ERR_CAPACITY, "cell table of the batch exceeds 2^32 entries");
+            cell_base[w + 1] = (uint32_t)base;
+            float bx0 = INFINITY, by0 = INFINITY, bx1 = -INFINITY, by1 = -INFINITY;
+            bool first = true;
+            for (uint32_t i = wstart[w]; i < wstart[w + 1]; ++i) {
+                if (W > 1) wid[i] = w;
+                if (first && !(bflags[i] & BF_STATIC)) {
+                    bflags[i] |= BF_FIRST_DYN;
+                    first = false;
+                }
+                const float x = cabs[i].x, y = cabs[i].y;
+                if (isfinite(x) && isfinite(y)) {
+                    bx0 = fminf(bx0, x);
+                    bx1 = fmaxf(bx1, x);
+                    by0 = fminf(by0, y);
+                    by1 = fmaxf(by1, y);
+                }
+            }
+            uint32_t* b0 = &wbbox[(size_t)w * 4];
+            b0[0] = f2ord_host(bx0);
+            b0[1] = f2ord_host(by0);
+            b0[2] = f2ord_host(bx1);
+            b0[3] = f2ord_host(by1);
+            uint32_t* b1 = &wbbox[((size_t)W + w) * 4];
+            b1[0] = b1[1] = 0xFFFFFFFFu;
+            b1[2] = b1[3] = 0u;
+        }
+        if (W == 1 && cell_base[1] < d.cells_cap) cell_base[1] = d.cells_cap;
+        d.n_worlds = W;
+        d.total_cells = cell_base[W];
+        cell_rows = d.total_cells;
+        HIP_TRY(dmalloc(&d.cell_base, W + 1));
+        HIP_TRY(dmalloc(&d.grids, W));
+        HIP_TRY(dmalloc(&d.wbbox, 8u * (size_t)W));
+        HIP_TRY(dmalloc(&d.wold_dt, W));
+        HIP_TRY(h2d(d.cell_base, cell_base, s));
+        HIP_TRY(h2d(d.wbbox, wbbox, s));
+        HIP_TRY(h2d(d.wold_dt, wold, s));
+        if (W > 1) {
+            HIP_TRY(dmalloc(&d.wid, nb));
+            HIP_TRY(h2d(d.wid, wid, s));
+        }
         // packed records in slot order; slot = dense index until the first rebuild
         for (int copy = 0; copy < 2; ++copy) {
             SphereArrays& q = copy ? d.s_alt : d.s;
@@ -826,8 +886,8 @@ int World::upload() {
             HIP_TRY(dmalloc(&d.oh_next[i], nb));
             HIP_TRY(dmalloc(&d.oh_bits[i], d.oh_size / 32u));
         }
-        HIP_TRY(dmalloc(&d.cell_count, (size_t)d.cells_cap + 1));
-        HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), s));
+        HIP_TRY(dmalloc(&d.cell_count, (size_t)d.total_cells + 1));
+        HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.total_cells + 1), s));
         HIP_TRY(dmalloc(&d.perm, nb));
         d.gather_pending = false;
         std::vector<float4> pp(nb), snap(nb);
@@ -920,14 +980,14 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.vals, nc));
     HIP_TRY(dmalloc(&d.keys_tmp, nc));
     HIP_TRY(dmalloc(&d.vals_tmp, nc));
-    HIP_TRY(dmalloc(&d.cell_start, (size_t)d.cells_cap + 1));
+    HIP_TRY(dmalloc(&d.cell_start, (size_t)cell_rows + 1));
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
     d.n_block_slots = (std::max<uint32_t>(nb, 1) + 255) / 256;
     HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
     HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
-    HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, d.cells_cap + 1)));
-    HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), d.stream));
+    HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, cell_rows + 1)));
+    HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)cell_rows + 1), d.stream));
     d.allocated = true;
 
     HIP_TRY(h2d(d.bc_start, bc_start, s));
@@ -1011,10 +1071,20 @@ int World::download() {
         HIP_TRY(d2h(cabs, d.c.abs, nc, s));
         for (uint32_t k = 0; k < nb; ++k) aid[k] = k;
     }
-    StepAccum acc;
-    HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    params_.old_dt = acc.old_dt;
+    if (path_ == StepPath::Sphere) {
+        std::vector<float> wold;
+        HIP_TRY(d2h(wold, d.wold_dt, d.n_worlds, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (!batch_start_.empty())
+            batch_old_dt_ = wold;
+        else
+            params_.old_dt = wold[0];
+    } else {
+        StepAccum acc;
+        HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        params_.old_dt = acc.old_dt;
+    }
     // in the sphere path slot k holds dense body aid[k] (and its collider)
     for (uint32_t k = 0; k < nb; ++k) {
         uint32_t dense = (path_ == StepPath::Sphere) ? aid[k] : k;
@@ -1134,8 +1204,7 @@ int World::check_errors() {
     return BG_OK;
 }
 
-int World::step(double delta, uint32_t n) {
-    if (n == 0) return BG_OK;
+int World::prepare_step() {
     if (params_.collisions_enabled && params_.use_spatial_hash)
         return fail(BG_ERR_UNSUPPORTED, "spatial collisions not supported right now (physics.rs:411-412)");
     int st = ensure_device();
@@ -1145,22 +1214,44 @@ int World::step(double delta, uint32_t n) {
         if (st) return st;
     }
     if (structure_error_) return fail(BG_ERR_PANIC, structure_msg_);
+    return BG_OK;
+}
+
+// Enqueues n steps (no host synchronisation; safe under stream capture).
+int World::launch_steps(double delta, uint32_t n) {
     for (uint32_t k = 0; k < n; ++k) {
         const float df = (float)delta;                             // physics.rs:80
         const float h = df / (float)params_.substeps;              // physics.rs:399
         for (uint32_t sub = 0; sub < params_.substeps; ++sub) {
             if (need_rebuild_ && params_.collisions_enabled) {
-                st = rebuild();
+                int st = rebuild();
                 if (st) return st;
             }
-            st = run_substep(h, fresh_);
+            int st = run_substep(h, fresh_);
             if (st) return st;
             fresh_ = false;
             stepped_since_upload_ = true;
         }
-        params_.time += delta;  // physics.rs:81
         if (++steps_since_rebuild_ >= rebuild_interval_) need_rebuild_ = true;
     }
+    return BG_OK;
+}
+
+// time += delta per step (physics.rs:81), for every world of a batch
+void World::advance_time(double delta, uint32_t n) {
+    for (uint32_t k = 0; k < n; ++k) {
+        params_.time += delta;
+        for (double& t : batch_time_) t += delta;
+    }
+}
+
+int World::step(double delta, uint32_t n) {
+    if (n == 0) return BG_OK;
+    int st = prepare_step();
+    if (st) return st;
+    st = launch_steps(delta, n);
+    if (st) return st;
+    advance_time(delta, n);
     host_stale_ = true;
     return check_errors();
 }

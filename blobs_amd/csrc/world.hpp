@@ -6,6 +6,8 @@
 
 #include <stdint.h>
 
+#include <hip/hip_runtime.h>
+
 #include <string>
 #include <utility>
 #include <vector>
@@ -31,6 +33,18 @@ struct HostConstraint {
 };
 
 enum class StepPath : uint32_t { None = 0, General = 1, Sphere = 2 };
+
+// One world flattened for a differentiable rollout (dense iteration order).
+struct DiffHostWorld {
+    std::vector<float2> pos, pos_old, vel, vreq, acc;
+    std::vector<float> mass, radius, gmod;
+    std::vector<uint32_t> flags;
+    std::vector<int32_t> sa, sb;  // spring endpoints, dense body index
+    std::vector<float> rest, k, c;
+    float old_dt = 1.0f, gx = 0.0f, gy = 0.0f;
+    uint32_t substeps = 8;
+    bool collisions = true;
+};
 
 struct Status {
     int code = BG_OK;
@@ -59,6 +73,17 @@ public:
     int step(double delta, uint32_t n);
     int fixed_step(double frame_time);
     int synchronize();
+    // n_chunks x chunk_steps steps, each chunk replayed from a captured hipGraph
+    int step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks);
+    // time candidate broadphase settings over real steps; keep the fastest
+    int autotune(double delta, uint32_t steps_per_trial, float* cell_scale, float* skin, uint32_t* interval);
+
+    // batches of single-sphere worlds (batch.cpp)
+    int init_batch(World* const* src, uint32_t n);
+    int batch_writeback(World* const* dst, uint32_t n);
+    uint32_t batch_size() const { return batch_start_.empty() ? 0u : (uint32_t)batch_start_.size() - 1; }
+    // differentiable rollouts (diff.cpp)
+    int export_diff(DiffHostWorld& out);
 
     // access
     int get_params(bg_params* p);
@@ -93,6 +118,11 @@ private:
     void mark_dirty() { dev_dirty_ = true; }
     int rebuild();
     int run_substep(float h, bool fresh);
+    int prepare_step();
+    int launch_steps(double delta, uint32_t n);
+    void advance_time(double delta, uint32_t n);
+    bool sphere_eligible() const;
+    void dense_slots(std::vector<uint32_t>& bodies, std::vector<uint32_t>& cols) const;
     int check_errors();
     int device_contacts(uint64_t* out);
     void free_device();
@@ -135,6 +165,10 @@ private:
     // broadphase drift of the current rebuild period
     double drift_x_ = 0, drift_y_ = 0, vdrift_x_ = 0, vdrift_y_ = 0, h_prev_ = 0;
     uint64_t contact_base_ = 0;
+    // batch: dense body range, old_dt and time of each source world
+    std::vector<uint32_t> batch_start_;
+    std::vector<float> batch_old_dt_;
+    std::vector<double> batch_time_;
     // profiling: event pairs recorded since the last collection, per stage
     bool profiling_ = false;
     std::vector<hipEvent_t> event_pool_;

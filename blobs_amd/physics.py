@@ -403,6 +403,50 @@ class Physics:
     def fixed_step(self, frame_time: float) -> None:
         self._chk(self._lib.bg_fixed_step(self._w, float(frame_time)))
 
+    def run_chunks(self, delta: float, chunk_steps: int, n_chunks: int) -> None:
+        """n_chunks x chunk_steps steps, each chunk a replayed hipGraph (same
+        results as step(delta, chunk_steps * n_chunks))."""
+        self._chk(self._lib.bg_batch_run(self._w, float(delta), int(chunk_steps), int(n_chunks)))
+
+    def autotune(self, delta: float, steps_per_trial: int = 2):
+        """Pick the fastest broadphase settings over real steps; returns
+        (cell_scale, skin, rebuild_interval)."""
+        cs, sk, iv = C.c_float(), C.c_float(), C.c_uint32()
+        self._chk(self._lib.bg_autotune(self._w, float(delta), int(steps_per_trial), C.byref(cs), C.byref(sk),
+                                        C.byref(iv)))
+        return cs.value, sk.value, iv.value
+
+    # ---- batches of single-sphere worlds
+    @classmethod
+    def batch(cls, worlds) -> "Physics":
+        """One device world stepping every world of `worlds` (each a
+        single-sphere Physics) in one launch per substep; bodies in world
+        order.  Results per world are identical to stepping it alone."""
+        worlds = list(worlds)
+        lib = L.load()
+        arr = (C.c_void_p * len(worlds))(*[w._w.value for w in worlds])
+        out = C.c_void_p()
+        code = lib.bg_batch_create(arr, len(worlds), C.byref(out))
+        L.check(code, None)
+        self = cls.__new__(cls)
+        self._lib = lib
+        self._w = out
+        self.springs = _SpringArena(self)
+        self.constraints = _ConstraintVec(self)
+        self._batch_worlds = worlds
+        return self
+
+    def batch_size(self) -> int:
+        n = C.c_uint32()
+        self._chk(self._lib.bg_batch_size(self._w, C.byref(n)))
+        return n.value
+
+    def writeback(self, worlds=None) -> None:
+        """Copy a batch's state back into its source worlds."""
+        worlds = list(worlds) if worlds is not None else self._batch_worlds
+        arr = (C.c_void_p * len(worlds))(*[w._w.value for w in worlds])
+        self._chk(self._lib.bg_batch_writeback(self._w, arr, len(worlds)))
+
     # ---- access (physics.rs:101-157)
     def rbd_count(self) -> int:
         n = C.c_uint32()

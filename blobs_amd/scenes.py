@@ -286,6 +286,17 @@ def batch_world_sizes(n_worlds=4096, lo=64, hi=16384, seed=4):
     return np.rint(rng.uniform(n_worlds, lo, hi, log=True)).astype(np.int64), rng
 
 
+def cfg4_scenes(n_worlds=4096, lo=64, hi=16384, seed=4, size_cap=None):
+    """The batched scene: n_worlds single-sphere worlds, N_w log-uniform in
+    [lo, hi], packing U[0.2, 0.5] per world, all drawn from one seeded
+    stream (sizes, then packings, then each world's bodies)."""
+    sizes, rng = batch_world_sizes(n_worlds, lo, hi, seed)
+    packs = rng.uniform(n_worlds, 0.2, 0.5)
+    if size_cap is not None:
+        sizes = np.minimum(sizes, size_cap)
+    return [cfg4_world(int(sizes[k]), float(packs[k]), rng) for k in range(n_worlds)]
+
+
 def cfg4_world(size, packing, seed_rng: Rng, vel_sigma=1.0):
     """One batched world: single spheres r~U[0.1,0.5] at the given packing."""
     side = box_side_for_packing(int(size), 0.1, 0.5, float(packing))
@@ -296,3 +307,44 @@ def cfg4_world(size, packing, seed_rng: Rng, vel_sigma=1.0):
                                       _f(pos), _f(vel), _f(rad))
     return _single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1.0 / 60.0, 1000,
                                 {"config": "batched_world", "box_side": side, "forced_overlaps": int(forced)})
+
+
+def cfg5_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0, kmax=1e3):
+    """One differentiable-rollout world: n single spheres r~U[0.5,1.0] at the
+    given packing (square box centred on the origin: position Verlet forms
+    velocities as p - p_old in f32, so small coordinates keep them
+    accurate), velocity requests N(0, vel_sigma),
+    n_springs springs between random bodies within distance 6 (rest =
+    initial distance, stiffness logU[kmin, kmax], damping 0), gravity on,
+    dt = 1/120."""
+    side = box_side_for_packing(int(n), 0.5, 1.0, float(packing))
+    pos = np.zeros((n, 2), np.float32)
+    vel = np.zeros((n, 2), np.float32)
+    rad = np.zeros(n, np.float32)
+    half = side / 2.0
+    forced = _scenes_lib().sg_spheres(rng.st, int(n), 0.5, 1.0, -half, -half, half, half, vel_sigma, 1, 200,
+                                      _f(pos), _f(vel), _f(rad))
+    sa = np.zeros(n_springs, np.int32)
+    sb = np.zeros(n_springs, np.int32)
+    rest = np.zeros(n_springs, np.float32)
+    made = _scenes_lib().sg_pairs(rng.st, int(n), _f(pos), int(n_springs), 6.0, _i(sa), _i(sb), _f(rest))
+    assert made == n_springs
+    k = rng.uniform(n_springs, kmin, kmax, log=True)
+    sc = _single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1.0 / 120.0, 200,
+                              {"config": "differentiable_world", "box_side": side, "forced_overlaps": int(forced),
+                               "spring_damping": 0.0})
+    sc.spring_a, sc.spring_b, sc.spring_rest = sa, sb, rest
+    sc.spring_k, sc.spring_c = k, np.zeros(n_springs, np.float32)
+    return sc
+
+
+def cfg5_scenes(n_worlds=256, n=2048, n_springs=1024, seed=5, **kw):
+    """The differentiable rollout: n_worlds worlds of n spheres + n_springs
+    springs, drawn from one seeded stream, plus one seeded target point per
+    world (uniform over the world's box)."""
+    rng = Rng(seed)
+    worlds = [cfg5_world(n, n_springs, rng, **kw) for _ in range(n_worlds)]
+    side = np.array([w.meta["box_side"] for w in worlds], np.float32)
+    u = rng.uniform(2 * n_worlds, -0.5, 0.5).reshape(n_worlds, 2)
+    targets = (u * side[:, None]).astype(np.float32)
+    return worlds, targets

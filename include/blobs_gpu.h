@@ -192,6 +192,29 @@ int bg_step(bg_world *w, double delta, uint32_t n_steps);
 int bg_fixed_step(bg_world *w, double frame_time);
 /* Wait for queued work on the world's stream. */
 int bg_synchronize(bg_world *w);
+/* n_chunks x chunk_steps calls of Physics::step(delta) (physics.rs:78-82),
+ * each chunk replayed from a hipGraph captured once (launch overhead paid
+ * once per capture, not per kernel).  Same results as bg_step with
+ * n_steps = chunk_steps * n_chunks.  The reference has no counterpart: it
+ * is the multi-step driver the batched benchmark loop needs. */
+int bg_batch_run(bg_world *w, double delta, uint32_t chunk_steps, uint32_t n_chunks);
+
+/* ---- batches of worlds (GPU-native; no counterpart in the reference) --
+ * A batch steps many independent `Physics` worlds (physics.rs:11-34 each)
+ * with one launch per substep.  Every source world must be a single-sphere
+ * world (each body has exactly one collider, inserted in the same order;
+ * no springs, joints or circle constraints) and all must share substeps,
+ * gravity and collisions_enabled.  The batch copies their state; each
+ * world keeps its own old_dt (Q1, physics.rs:338-340), time and grid, and
+ * contacts never cross worlds, so every world evolves bit-identically to
+ * stepping it alone.  The batch is itself a bg_world: bg_step,
+ * bg_batch_run, bg_autotune and the getters work on it (bodies in world
+ * order).  Destroy it with bg_world_destroy. */
+int bg_batch_create(bg_world *const *worlds, uint32_t n_worlds, bg_world **out);
+/* Copy the batch's state back into its source worlds (same order). */
+int bg_batch_writeback(bg_world *batch, bg_world *const *worlds, uint32_t n_worlds);
+/* Number of worlds in a batch (0 for a plain world). */
+int bg_batch_size(bg_world *w, uint32_t *out);
 
 /* ---- state access ---------------------------------------------------- */
 
@@ -228,6 +251,41 @@ int bg_last_error(bg_world *w, char *buf, size_t len);
 int bg_set_broadphase(bg_world *w, float cell_scale, float skin, uint32_t rebuild_interval);
 /* Which device path the last step ran: 0 none, 1 general, 2 fused spheres. */
 int bg_step_path(bg_world *w, uint32_t *out);
+/* Broadphase autotuner: runs steps_per_trial real steps of delta under each
+ * of a fixed list of (cell_scale, skin, rebuild_interval) candidates, timed
+ * with HIP events, and keeps the fastest (reported through the non-NULL
+ * out pointers).  The steps advance the simulation as bg_step would; the
+ * choice changes speed only. */
+int bg_autotune(bg_world *w, double delta, uint32_t steps_per_trial, float *cell_scale, float *skin,
+                uint32_t *rebuild_interval);
+
+/* ---- differentiable rollouts (GPU-native; no counterpart in the reference)
+ * A rollout batches single-sphere worlds with springs (one centred,
+ * non-sensor collider per body inserted in body order; no joints or circle
+ * constraints; at most 2,048 bodies per world; worlds share gravity,
+ * substeps and collisions_enabled) and runs n_steps calls of
+ * Physics::step(delta) (physics.rs:78-82) on copies of their state, keeping
+ * a checkpoint every `checkpoint_every` steps.  Parameters: the initial
+ * velocity requests (one Vec2 per body, rigid_body.rs:182-184) and the
+ * spring stiffnesses (springs.rs:16-22).  The loss of world w is
+ * |centroid(final positions) - target_w|^2.  The adjoint differentiates
+ * the step with every substep's contact pair set held fixed, recomputing
+ * each segment from its checkpoint; damped springs are refused by it. */
+typedef struct bg_rollout bg_rollout;
+int bg_rollout_create(bg_world *const *worlds, uint32_t n_worlds, const float *targets_xy, uint32_t checkpoint_every,
+                      bg_rollout **out);
+int bg_rollout_destroy(bg_rollout *r);
+int bg_rollout_sizes(bg_rollout *r, uint32_t *n_worlds, uint32_t *n_bodies, uint32_t *n_springs);
+/* Replace the parameters (NULL keeps the current ones); bodies and springs
+ * in world order, then iteration order. */
+int bg_rollout_set_params(bg_rollout *r, const float *velocity_request, const float *stiffness);
+/* Forward rollout from the worlds' initial state; loss per world (may be NULL). */
+int bg_rollout_forward(bg_rollout *r, double delta, uint32_t n_steps, float *loss_per_world);
+/* Gradients of the last forward: of the centroid loss summed over worlds,
+ * or of sum(dloss_dfinal_position . final position) when that is given. */
+int bg_rollout_backward(bg_rollout *r, const float *dloss_dfinal_position, float *grad_velocity_request,
+                        float *grad_stiffness);
+int bg_rollout_final_positions(bg_rollout *r, float *out);
 
 /* ---- profiling ------------------------------------------------------- */
 

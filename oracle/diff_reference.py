@@ -1,0 +1,242 @@
+"""FP64 reference for differentiable rollouts (torch on the CPU).
+
+TEST INFRASTRUCTURE ONLY -- the checker for the GPU adjoint.  Only tests/
+and bench.py's reference leg use it; nothing in blobs_amd/ does.
+
+A restatement of blobs' substep for single-sphere worlds with springs, in
+float64 torch ops so autograd gives the exact gradient of the same map the
+GPU adjoint differentiates:
+
+  gravity + springs  physics.rs:369-375, springs.rs:25-47 (F = u (k(L-rest) - D))
+  contacts           physics.rs:241-317 (snapshot = positions at the substep
+                     start; A = the higher collider index; rho = 1 - mA/(mA+mB))
+  update_objects     physics.rs:326-358 (velocity request consumed once,
+                     Q1 ratio h/old_dt for the first dynamic body)
+
+The contact pair set of each substep is decided on the detached positions
+(reference predicate dist < r_a + r_b) and can be recorded, then replayed
+unchanged ("contact set held fixed") for finite differences.  Checked in
+tests against the oracle's FP64 twin (trajectory) and against central
+finite differences (gradients).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class RefWorlds:
+    """Concatenated worlds (arrays from blobs_amd.scenes.Scene objects)."""
+
+    def __init__(self, scenes, targets):
+        n = [s.n_bodies for s in scenes]
+        self.b0 = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+        cat = lambda f: np.concatenate([f(s) for s in scenes])
+        self.p0 = torch.tensor(cat(lambda s: s.pos), dtype=torch.float64)
+        self.vreq0 = torch.tensor(cat(lambda s: s.vreq), dtype=torch.float64)
+        self.radius = torch.tensor(cat(lambda s: s.col_r), dtype=torch.float64)
+        # mass = 2 * sum(2 r) over the collider handle pushed twice (Q2): 4 r, or 2 * mass_override
+        mo = cat(lambda s: np.where(s.col_has_mo != 0, s.col_mass_override, 2.0 * s.col_r))
+        self.mass = torch.tensor(2.0 * np.asarray(mo, np.float64), dtype=torch.float64)
+        self.static = torch.tensor(cat(lambda s: s.body_type == 1))
+        self.has_vreq = torch.tensor(cat(lambda s: s.has_vreq != 0))
+        self.gmod = torch.tensor(cat(lambda s: s.gravity_mod), dtype=torch.float64)
+        self.world = torch.tensor(np.concatenate([np.full(k, w) for w, k in enumerate(n)]), dtype=torch.int64)
+        off = self.b0[:-1]
+        self.sa = torch.tensor(np.concatenate([s.spring_a + o for s, o in zip(scenes, off)]), dtype=torch.int64)
+        self.sb = torch.tensor(np.concatenate([s.spring_b + o for s, o in zip(scenes, off)]), dtype=torch.int64)
+        self.rest = torch.tensor(cat(lambda s: s.spring_rest), dtype=torch.float64)
+        self.k0 = torch.tensor(cat(lambda s: s.spring_k), dtype=torch.float64)
+        self.c = torch.tensor(cat(lambda s: s.spring_c), dtype=torch.float64)
+        self.gravity = torch.tensor(scenes[0].gravity, dtype=torch.float64)
+        self.targets = torch.tensor(np.asarray(targets, np.float64).reshape(len(scenes), 2))
+        self.n_worlds = len(scenes)
+        dyn = (~self.static).numpy()
+        self.first_dyn = np.array([b + int(np.argmax(dyn[b:e])) if dyn[b:e].any() else -1
+                                   for b, e in zip(self.b0[:-1], self.b0[1:])])
+
+    def pairs(self, p):
+        """Reference contact pairs (A, B), A > B, same world, on positions p."""
+        out = []
+        pd = p.detach()
+        for w in range(self.n_worlds):
+            b, e = int(self.b0[w]), int(self.b0[w + 1])
+            x = pd[b:e]
+            d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+            rs = self.radius[b:e, None] + self.radius[None, b:e]
+            hit = (d2.sqrt() < rs)
+            ii, jj = torch.nonzero(torch.tril(hit, diagonal=-1), as_tuple=True)
+            out.append(torch.stack([ii + b, jj + b], 1))
+        return torch.cat(out) if out else torch.zeros((0, 2), dtype=torch.int64)
+
+    def rollout(self, vreq, k, delta, n_steps, substeps=8, old_dt0=1.0, pairs_in=None, record=None,
+                collisions=True, f64_step=False):
+        """Final positions after n_steps; pairs_in replays recorded pair sets.
+        The substep length is the f32 h of physics.rs:80,399 (what the GPU
+        uses), or delta / substeps in f64 with f64_step (the FP64 twin's)."""
+        if f64_step:
+            h = float(delta) / substeps
+        else:
+            h = float(np.float32(np.float32(delta) / np.float32(substeps)))
+        p = self.p0.clone()
+        q = self.p0.clone()
+        v = torch.zeros_like(p)
+        dyn = (~self.static).to(torch.float64)[:, None]
+        first = torch.zeros(p.shape[0], dtype=torch.float64)
+        first[[f for f in self.first_dyn if f >= 0]] = 1.0
+        for gsub in range(n_steps * substeps):
+            acc = (self.gravity[None, :] * self.gmod[:, None]) * dyn
+            d = p[self.sb] - p[self.sa]
+            L = d.norm(dim=1, keepdim=True)
+            u = d / L
+            rv = v[self.sa] - v[self.sb]
+            D = (self.c[:, None] * (rv * u).sum(1, keepdim=True)) * u
+            F = u * (k[:, None] * (L - self.rest[:, None]) - D)
+            acc = acc.index_add(0, self.sa, F / self.mass[self.sa, None] * dyn[self.sa])
+            acc = acc.index_add(0, self.sb, -F / self.mass[self.sb, None] * dyn[self.sb])
+            p1 = p
+            if collisions:
+                pr = pairs_in[gsub] if pairs_in is not None else self.pairs(p)
+                if record is not None:
+                    record.append(pr)
+                if pr.shape[0]:
+                    A, B = pr[:, 0], pr[:, 1]
+                    axis = p[A] - p[B]
+                    dist = axis.norm(dim=1, keepdim=True)
+                    n = axis / dist
+                    delta_ = self.radius[A, None] + self.radius[B, None] - dist
+                    mA, mB = self.mass[A, None], self.mass[B, None]
+                    rho = 1.0 - mA / (mA + mB)
+                    corr = torch.zeros_like(p)
+                    corr = corr.index_add(0, A, rho * delta_ * n)
+                    corr = corr.index_add(0, B, -(1.0 - rho) * delta_ * n)
+                    p1 = p + corr
+            if gsub == 0:
+                q = torch.where(self.has_vreq[:, None], p1 - vreq * h, q)
+                ratio = (h / old_dt0) * first[:, None] + (1.0 - first[:, None])
+            else:
+                ratio = 1.0
+            disp = (p1 - q) * ratio
+            pn = p1 + (disp + acc * h * h)
+            p = dyn * pn + (1.0 - dyn) * p1
+            q = p1
+            v = dyn * disp / h
+        return p
+
+    def pairs_f32(self, p32: np.ndarray) -> np.ndarray:
+        """Reference contact pairs (A, B), A > B, decided in f32 exactly as
+        physics.rs:264-269 on f32 positions (axis, dot, sqrt, dist < r_a + r_b)."""
+        out = []
+        rad = self.radius.numpy().astype(np.float32)
+        for w in range(self.n_worlds):
+            b, e = int(self.b0[w]), int(self.b0[w + 1])
+            x = p32[b:e].astype(np.float32)
+            ax = x[:, None, 0] - x[None, :, 0]
+            ay = x[:, None, 1] - x[None, :, 1]
+            dist = np.sqrt(ax * ax + ay * ay)
+            rs = rad[b:e, None] + rad[None, b:e]
+            ii, jj = np.nonzero(np.tril(dist < rs, -1))
+            out.append(np.stack([ii + b, jj + b], 1))
+        return np.concatenate(out) if out else np.zeros((0, 2), np.int64)
+
+    def tape_adjoint(self, tape, pairs, final_p, delta, substeps=8, dl_dfinal=None):
+        """FP64 reverse pass along a recorded trajectory: tape[g] = positions
+        at the start of substep g (the contact snapshot), pairs[g] = its
+        contact pairs.  Returns (d loss / d vreq, d loss / d k) of the
+        centroid loss (or of sum(dl_dfinal . final positions)).  Written
+        from the substep map's derivative, independently of autograd."""
+        h = float(np.float32(np.float32(delta) / np.float32(substeps)))
+        m = self.mass.numpy()
+        r = self.radius.numpy()
+        sa, sb = self.sa.numpy(), self.sb.numpy()
+        rest, k = self.rest.numpy(), self.k0.numpy()
+        dyn = ~self.static.numpy()
+        n = m.shape[0]
+        fp = np.asarray(final_p, np.float64)
+        if dl_dfinal is None:
+            lp = np.zeros((n, 2))
+            for w in range(self.n_worlds):
+                b, e = int(self.b0[w]), int(self.b0[w + 1])
+                lp[b:e] = 2.0 * (fp[b:e].mean(0) - self.targets[w].numpy()) / (e - b)
+        else:
+            lp = np.asarray(dl_dfinal, np.float64).copy()
+        lq = np.zeros((n, 2))
+        gv = np.zeros((n, 2))
+        gk = np.zeros(k.shape[0])
+        first = np.zeros(n, bool)
+        first[[f for f in self.first_dyn if f >= 0]] = True
+        has_v = self.has_vreq.numpy()
+        for g in range(len(tape) - 1, -1, -1):
+            s = np.asarray(tape[g], np.float64)
+            ratio = np.where(first & (g == 0), h / 1.0, 1.0)[:, None]
+            if g == 0:
+                cons = (dyn & has_v)[:, None]
+                lpp = np.where(cons, lp + lq, (1.0 + ratio) * lp + lq)
+                lqin = np.where(cons, 0.0, -ratio * lp)
+                gv += np.where(cons, h * ratio * lp, 0.0)
+            else:
+                lpp = (1.0 + ratio) * lp + lq
+                lqin = -ratio * lp
+            lpp = np.where(dyn[:, None], lpp, lp + lq)
+            lqin = np.where(dyn[:, None], lqin, 0.0)
+            lam_am = np.where(dyn[:, None], h * h * lp / m[:, None], 0.0)
+            ls = np.zeros((n, 2))
+            pr = np.asarray(pairs[g]).reshape(-1, 2)
+            if pr.shape[0]:
+                for i, j in ((pr[:, 0], pr[:, 1]), (pr[:, 1], pr[:, 0])):
+                    a = s[i] - s[j]
+                    d = np.linalg.norm(a, axis=1, keepdims=True)
+                    rs = (r[i] + r[j])[:, None]
+                    wi = (m[j] / (m[i] + m[j]))[:, None]
+                    wj = (m[i] / (m[i] + m[j]))[:, None]
+                    mu = wi * lpp[i] - wj * lpp[j]
+                    nn = a / d
+                    nm = (nn * mu).sum(1, keepdims=True)
+                    np.add.at(ls, i, (rs / d) * (mu - nn * nm) - mu)
+            dv = s[sb] - s[sa]
+            L = np.linalg.norm(dv, axis=1, keepdims=True)
+            u = dv / L
+            lF = lam_am[sa] - lam_am[sb]
+            uf = (u * lF).sum(1, keepdims=True)
+            KlF = (k[:, None] * (1.0 - rest[:, None] / L)) * (lF - u * uf) + k[:, None] * u * uf
+            np.add.at(ls, sa, -KlF)
+            np.add.at(ls, sb, KlF)
+            gk += ((L - rest[:, None]) * uf)[:, 0]
+            lp = lpp + ls
+            lq = lqin
+        return gv, gk
+
+    def loss(self, final_p):
+        tot = final_p.new_zeros(())
+        for w in range(self.n_worlds):
+            b, e = int(self.b0[w]), int(self.b0[w + 1])
+            cen = final_p[b:e].mean(0)
+            tot = tot + ((cen - self.targets[w]) ** 2).sum()
+        return tot
+
+    def gradients(self, delta, n_steps, substeps=8, vreq=None, k=None, record=None):
+        """(loss, d loss / d vreq [n,2], d loss / d k [s]) by autograd in float64."""
+        vr = (self.vreq0 if vreq is None else torch.as_tensor(vreq, dtype=torch.float64)).clone().requires_grad_(True)
+        kk = (self.k0 if k is None else torch.as_tensor(k, dtype=torch.float64)).clone().requires_grad_(True)
+        L = self.loss(self.rollout(vr, kk, delta, n_steps, substeps, record=record))
+        gv, gk = torch.autograd.grad(L, [vr, kk])
+        return float(L.detach()), gv.numpy(), gk.numpy()
+
+
+def f32_tape(scenes, delta, n_steps, substeps=8):
+    """The f32 trajectory of each world (the oracle's f32 restatement, one
+    substep at a time): positions at the start of every substep, and the
+    final positions, concatenated over worlds."""
+    from .oracle import OracleWorld
+    h = float(np.float32(np.float32(delta) / np.float32(substeps)))
+    tapes, finals = [], []
+    for sc in scenes:
+        o = OracleWorld(sc, np.float32, substeps=1)
+        tr = []
+        for _ in range(n_steps * substeps):
+            tr.append(o.pos.astype(np.float32).copy())
+            o.step(h, 1)
+        tapes.append(tr)
+        finals.append(o.pos.astype(np.float32).copy())
+    tape = [np.concatenate([t[g] for t in tapes]) for g in range(n_steps * substeps)]
+    return tape, np.concatenate(finals)

@@ -180,3 +180,40 @@ def test_rigid_body_builder_defaults():
     h = p.insert_rbd(b)
     got = p.get_rbd(h)
     assert got.position == (1.0, 2.0) and got.calculated_mass == 1.0
+
+
+def test_batch_construction_on_host():
+    """bg_batch_create copies single-sphere worlds (host only) and refuses
+    worlds that are not, or whose parameters differ."""
+    scs = scenes.cfg4_scenes(3, lo=64, hi=200, seed=48)
+    src = [sc.build_physics()[0] for sc in scs]
+    b = Physics.batch(src)
+    assert b.batch_size() == 3
+    assert b.rbd_count() == sum(sc.n_bodies for sc in scs)
+    assert src[0].batch_size() == 0
+    pos = b.bodies()["position"]
+    assert np.array_equal(pos, np.concatenate([sc.pos for sc in scs]))
+    b.writeback(src)  # nothing stepped: unchanged
+    assert np.array_equal(src[1].bodies()["position"], scs[1].pos)
+    # substeps differ
+    c = scs[0].build_physics()[0]
+    c.substeps = 4
+    with pytest.raises(L.BlobsError) as e:
+        Physics.batch([src[0], c])
+    assert e.value.code == L.BG_ERR_INVALID_ARGUMENT
+    # a world with a spring is not a single-sphere world
+    d = scs[1].build_physics()[0]
+    h = d.bodies()["handles"]
+    d.springs.insert(Spring(int(h[0]), int(h[1]), 1.0, 10.0, 0.0))
+    with pytest.raises(L.BlobsError) as e:
+        Physics.batch([src[0], d])
+    assert e.value.code == L.BG_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_batch_run_without_device_fails_loudly():
+    sc = scenes.cfg4_scenes(2, lo=64, hi=100, seed=49)
+    b = Physics.batch([s.build_physics()[0] for s in sc])
+    with pytest.raises(L.BlobsError) as e:
+        b.run_chunks(1 / 60, 10, 2)
+    assert e.value.code == L.BG_ERR_NO_DEVICE
