@@ -89,6 +89,86 @@ def traffic_per_launch(n_bodies: int):
     return rec.get("traffic_bytes_per_launch")
 
 
+def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) -> dict:
+    """The uneven batch: n_worlds single-sphere worlds (N_w log-uniform in
+    [64, 16384]), step(1/60) in chunk_steps-step hipGraph chunks; whole-job
+    sphere-steps/s over the timed chunks (inputs resident in HBM)."""
+    from blobs_amd import scenes
+    from blobs_amd.physics import Physics
+    t0 = time.perf_counter()
+    scs = scenes.cfg4_scenes(n_worlds)
+    worlds = [sc.build_physics()[0] for sc in scs]
+    batch = Physics.batch(worlds)
+    n_bodies = batch.rbd_count()
+    del worlds
+    log(f"batched: {n_worlds} worlds, {n_bodies} spheres built in {time.perf_counter() - t0:.1f} s")
+    dt = scs[0].dt
+    batch.run_chunks(dt, chunk_steps, 1)  # upload, first grid, one-off inputs
+    tuned = None
+    if autotune:
+        t0 = time.perf_counter()
+        cs, sk, iv = batch.autotune(dt, 2)
+        tuned = {"cell_scale": cs, "skin": sk, "rebuild_interval": iv}
+        log(f"batched: autotune -> {tuned} ({time.perf_counter() - t0:.1f} s)")
+    batch.run_chunks(dt, chunk_steps, 1)  # graph capture warm-up
+    batch._lib.bg_synchronize(batch._w)
+    t0 = time.perf_counter()
+    batch.run_chunks(dt, chunk_steps, n_chunks)  # synchronizes at the end
+    elapsed = time.perf_counter() - t0
+    steps = chunk_steps * n_chunks
+    if not np.isfinite(batch.bodies()["position"]).all():
+        raise RuntimeError("non-finite positions in the batched run")
+    rec = {
+        "workload": f"{n_worlds} uneven worlds (N_w logU[64,16384], r U[0.1,0.5], packing U[0.2,0.5]), step(1/60)",
+        "worlds": n_worlds, "bodies": n_bodies, "steps": steps, "chunk_steps": chunk_steps,
+        "value": n_bodies * steps / elapsed, "unit": "sphere-steps/s", "ms_per_step": 1000.0 * elapsed / steps,
+        "autotuned": tuned,
+    }
+    log(f"batched: {steps} steps in {elapsed:.2f} s -> {rec['value']:.4g} sphere-steps/s")
+    return rec
+
+
+def differentiable_leg(n_worlds: int, n_steps: int, ck_every: int) -> dict:
+    """Forward and forward+adjoint of the differentiable rollout: n_worlds
+    worlds of 2,048 spheres + 1,024 springs, step(1/120), checkpoint every
+    ck_every steps; gradients of the centroid loss w.r.t. initial velocity
+    requests and stiffnesses."""
+    from blobs_amd import scenes
+    from blobs_amd.rollout import Rollout
+    t0 = time.perf_counter()
+    ws, targets = scenes.cfg5_scenes(n_worlds)
+    r = Rollout([w.build_physics()[0] for w in ws], targets, checkpoint_every=ck_every)
+    log(f"differentiable: {n_worlds} worlds built in {time.perf_counter() - t0:.1f} s")
+    dt = ws[0].dt
+    r.forward(dt, n_steps)
+    r.backward()  # warm-up (buffers, code objects)
+    t0 = time.perf_counter()
+    loss = r.forward(dt, n_steps)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gv, gk = r.backward()
+    t_bwd = time.perf_counter() - t0
+    fin_v, fin_k = np.isfinite(gv), np.isfinite(gk)
+    rec = {
+        "workload": f"{n_worlds} worlds x (2048 spheres + 1024 springs), step(1/120) x {n_steps}, "
+                    f"checkpoint every {ck_every}",
+        "bodies": r.n_bodies, "springs": r.n_springs, "steps": n_steps,
+        "forward_s": t_fwd, "forward_adjoint_s": t_fwd + t_bwd,
+        "fwd_adj_over_fwd": (t_fwd + t_bwd) / t_fwd,
+        "forward_steps_per_s": n_steps / t_fwd, "fwd_adj_steps_per_s": n_steps / (t_fwd + t_bwd),
+        "fwd_adj_sphere_steps_per_s": r.n_bodies * n_steps / (t_fwd + t_bwd),
+        "loss": float(loss.sum()),
+        # the fixed-pair-set derivative of the contact projection grows
+        # exponentially with the horizon (1e4-1e33 after 200 steps in the
+        # FP64 model), so long rollouts overflow f32 in some worlds
+        "grad_finite_fraction": float((fin_v.sum() + fin_k.sum()) / (fin_v.size + fin_k.size)),
+        "grad_max_abs_finite": float(max(np.abs(gv[fin_v]).max(initial=0.0), np.abs(gk[fin_k]).max(initial=0.0))),
+    }
+    log(f"differentiable: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms, "
+        f"ratio {rec['fwd_adj_over_fwd']:.2f}")
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +182,11 @@ def main() -> int:
     ap.add_argument("--skin", type=float, default=None)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (hardware byte-counter calibration)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-differentiable", action="store_true")
+    ap.add_argument("--batch-worlds", type=int, default=4096)
+    ap.add_argument("--batch-chunks", type=int, default=100)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -193,6 +278,12 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if not args.no_batched:
+        b = batched_leg(10, args.batch_chunks, args.batch_worlds, not args.no_autotune)
+        b["vs_plain"] = b["value"] / value
+        line["batched"] = b
+    if not args.no_differentiable:
+        line["differentiable"] = differentiable_leg(256, 200, 20)
     if not args.no_cpu:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)

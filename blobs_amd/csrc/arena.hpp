@@ -82,7 +82,10 @@ public:
     T& at_slot(uint32_t s) { return slots_[s].value; }
     const T& at_slot(uint32_t s) const { return slots_[s].value; }
     uint64_t handle_at(uint32_t s) const { return make_handle(s, slots_[s].gen); }
-    void reserve(size_t n) { slots_.reserve(n); }
+    // geometric growth: repeated small bulk inserts stay linear overall
+    void reserve(size_t n) {
+        if (n > slots_.capacity()) slots_.reserve(n > 2 * slots_.capacity() ? n : 2 * slots_.capacity());
+    }
 
 private:
     std::vector<Slot> slots_;

@@ -83,16 +83,18 @@ int World::init_batch(World* const* src, uint32_t n) {
     batch_old_dt_.clear();
     batch_time_.clear();
     std::vector<uint32_t> bs, cs;
+    size_t total = 0;
+    for (uint32_t w = 0; w < n; ++w) total += src[w]->bodies_.len();
+    bodies_.reserve(total);
+    colliders_.reserve(total);
     for (uint32_t w = 0; w < n; ++w) {
         const World* s = src[w];
         s->dense_slots(bs, cs);
-        bodies_.reserve(bodies_.capacity_slots() + bs.size());
-        colliders_.reserve(colliders_.capacity_slots() + cs.size());
         for (size_t k = 0; k < bs.size(); ++k) {
-            HostBody hb = s->bodies_.at_slot(bs[k]);
-            const size_t dup = hb.colliders.size();
-            hb.colliders.clear();
-            hb.joints.clear();
+            const HostBody& sb = s->bodies_.at_slot(bs[k]);
+            const size_t dup = sb.colliders.size();
+            HostBody hb;
+            hb.f = sb.f;
             const bg_handle bh = bodies_.insert(hb);
             HostCollider hc = s->colliders_.at_slot(cs[k]);
             hc.f.parent = bh;

@@ -85,6 +85,8 @@ struct DeviceWorld {
     GridParams *grids = nullptr;    // [n_worlds]
     uint32_t *wbbox = nullptr;      // [2][n_worlds][4] f2ord(min x, min y, max x, max y) per parity
     float *wold_dt = nullptr;       // [n_worlds] Physics::old_dt of each world (Q1)
+    float *wstat = nullptr;         // [n_worlds][8] rebuild statistics (sums; see k_world_stats)
+    float2 *wvel = nullptr;         // [n_worlds] mean velocity of the dynamic bodies at the last rebuild
     uint32_t total_cells = 0;       // cell_base[n_worlds]
     // springs
     int32_t *sp_a = nullptr, *sp_b = nullptr;
@@ -122,6 +124,7 @@ struct SubstepArgs {
     float drift_prev_x, drift_prev_y;  // common drift at the previous snapshot (broadphase query centre)
     float drift_x, drift_y;            // common drift after this substep
     float skin;         // sphere path: displacement since the rebuild beyond which a body is an outlier
+    float t_prev, t_cur;  // sphere path: time since the rebuild at the previous / this snapshot
     int32_t first_dyn;  // dense index of the first non-static body (Q1), -1 if none
     uint32_t fresh;     // first substep after an upload: user acc/torque are live
     uint32_t collisions;

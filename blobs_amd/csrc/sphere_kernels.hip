@@ -13,13 +13,15 @@
 // leaves its world's cells, so pairs never cross worlds.
 //
 // Broadphase.  A rebuild fits each world's grid to the bounding box of its
-// snapshots over the previous period, counts bodies per cell with atomic
+// snapshots over the previous period (clipped to the centroid +- 4 sigma;
+// bodies outside are clamped into border cells), counts bodies per cell with atomic
 // ranks, scans the counts into the cell table and scatters the snapshot
 // records and the permutation.  The first substep after it gathers the body
 // records through the permutation and writes them back in cell order, so
 // the reordering costs no separate pass.  Between rebuilds the sorted grid
 // keeps the rebuild snapshots (x0); a body whose snapshot moves more than
-// `skin` from x0 (after removing the common gravity drift) becomes an
+// `skin` from x0 (after removing its world's common motion: the gravity
+// drift plus the mean velocity of its dynamic bodies at the rebuild) becomes an
 // outlier and is, from then on, re-inserted every substep into a coarse
 // hashed cell table with an occupancy bitmap.  A query scans the sorted grid
 // with radius r + r_max + skin (skipping outliers) and the outlier table
@@ -49,6 +51,7 @@ struct SphereSubstep {
     const GridParams* __restrict__ grids;
     uint32_t* wbbox;  // this substep's parity: [n_worlds][4]
     float* wold_dt;
+    const float2* __restrict__ wvel;
     const uint32_t* __restrict__ oh_head;  // outlier table read this substep
     const uint32_t* __restrict__ oh_next;
     const uint32_t* __restrict__ oh_bits;
@@ -163,6 +166,10 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             if (GROUPS) ga = k.src.groups[g_own];
             const bool sen_a = SENSORS ? (fl & BF_SENSOR) != 0u : false;
             const uint32_t nout = __hip_atomic_load(&k.accum->n_outliers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // common motion of the world since the rebuild at the previous snapshot (the
+            // same expression as the skin test that wrote it)
+            const float2 vw = k.wvel[w];
+            const float dpx = a.drift_prev_x + vw.x * a.t_prev, dpy = a.drift_prev_y + vw.y * a.t_prev;
             // reference predicate (physics.rs:255-269) against the slot-t snapshot
             auto test = [&](uint32_t t, const float4 sb, auto&& sink) {
                 if (GROUPS) {
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             };
             auto enumerate = [&](auto&& sink) {
                 // sorted grid: four candidates per load batch (clamped indices, no branches)
-                grid_rows(g, cs, k.nb, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, ra + a.rmax + a.skin,
+                grid_rows(g, cs, k.nb, xa.x - dpx, xa.y - dpy, ra + a.rmax + a.skin,
                           [&](uint32_t t0, uint32_t t1) {
                               for (uint32_t t = t0; t < t1; t += 4) {
                                   float4 sb[4];
@@ -355,7 +362,9 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         bool out = was_out;
         if (a.collisions && !out && finite_x) {
             const float2 x0 = k.x0[s];
-            const float ex = (x.x - x0.x) - a.drift_x, ey = (x.y - x0.y) - a.drift_y;
+            const float2 vw = k.wvel[w];
+            const float dx = a.drift_x + vw.x * a.t_cur, dy = a.drift_y + vw.y * a.t_cur;
+            const float ex = (x.x - x0.x) - dx, ey = (x.y - x0.y) - dy;
             out = !(sqrtf(ex * ex + ey * ey) <= a.skin);
             new_outliers += out ? 1u : 0u;
         }
@@ -391,23 +400,95 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
 
 // ---- rebuild: one-pass counting sort by cell ------------------------------
 
-// Grid of every world, fitted to the bounding box of its snapshots over the
-// period that ends here; the other parity's boxes restart empty.
+// Per-world sums over the current snapshot for the next grid and drift:
+// [0,1] velocity of dynamic bodies, [2] their count, [3,4] position,
+// [5,6] squared position, [7] count.  Float atomics: the sums' rounding
+// order varies, which moves grid bounds and the drift model only -- never
+// a result (any grid and drift give the same pair set).
+template <bool BATCHED>
+__global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restrict__ snap,
+                                                          const float2* __restrict__ vel,
+                                                          const uint32_t* __restrict__ bflags, uint32_t nb,
+                                                          const uint32_t* __restrict__ wid, float* wstat) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < nb;
+    const uint32_t w = BATCHED ? wid[live ? i : nb - 1] : 0u;
+    float v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = 0.0f;
+    if (live) {
+        const float4 sn = snap[i];
+        if (finitef(sn.x) && finitef(sn.y)) {
+            if (!(bflags[i] & BF_STATIC)) {
+                const float2 vv = vel[i];
+                if (finitef(vv.x) && finitef(vv.y)) {
+                    v[0] = vv.x;
+                    v[1] = vv.y;
+                    v[2] = 1.0f;
+                }
+            }
+            v[3] = sn.x;
+            v[4] = sn.y;
+            v[5] = sn.x * sn.x;
+            v[6] = sn.y * sn.y;
+            v[7] = 1.0f;
+        }
+    }
+    const uint32_t w0 = __shfl(w, 0, BG_WAVE);
+    float* dst = wstat + (size_t)w * 8;
+    if (__all(w == w0)) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            for (int o = 32; o > 0; o >>= 1) v[t] += __shfl_xor(v[t], o, BG_WAVE);
+        if ((threadIdx.x & (BG_WAVE - 1)) == 0 && v[7] > 0.0f)
+            for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
+    } else if (v[7] > 0.0f) {
+        for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
+    }
+}
+
+// Grid of every world over its snapshots: the bounding box accumulated
+// over the period that ends here, clipped to the centroid +- 4 sigma so a
+// few far-flung bodies (clamped into border cells) do not coarsen the cells
+// of the rest; the mean velocity of its dynamic bodies (drift model); the
+// other parity's boxes restart empty.
 __global__ __launch_bounds__(kThreads) void k_world_grids(uint32_t n_worlds, uint32_t* wbbox, uint32_t parity,
                                                           float cell, const uint32_t* __restrict__ cell_base,
-                                                          GridParams* grids, StepAccum* accum) {
+                                                          const float* __restrict__ wstat, GridParams* grids,
+                                                          float2* wvel, StepAccum* accum) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w == 0) {
         accum->dmax = 0u;
         accum->n_outliers = 0u;
     }
     if (w >= n_worlds) return;
-    grids[w] = grid_from_bbox(wbbox + ((size_t)parity * n_worlds + w) * 4, cell, cell_base[w + 1] - cell_base[w]);
-    uint32_t* nb = wbbox + ((size_t)(parity ^ 1u) * n_worlds + w) * 4;
-    nb[0] = 0xFFFFFFFFu;
-    nb[1] = 0xFFFFFFFFu;
-    nb[2] = 0u;
-    nb[3] = 0u;
+    const float* st = wstat + (size_t)w * 8;
+    uint32_t bb[4];
+    const uint32_t* src = wbbox + ((size_t)parity * n_worlds + w) * 4;
+    for (int t = 0; t < 4; ++t) bb[t] = src[t];
+    if (st[7] > 0.0f && bb[0] <= bb[2] && bb[1] <= bb[3]) {
+        const float n = st[7];
+        const float mx = st[3] / n, my = st[4] / n;
+        const float sx = sqrtf(fmaxf(st[5] / n - mx * mx, 0.0f)), sy = sqrtf(fmaxf(st[6] / n - my * my, 0.0f));
+        const float mg = 2.0f * cell;
+        if (finitef(mx) && finitef(sx)) {
+            bb[0] = max(bb[0], f2ord(mx - 4.0f * sx - mg));
+            bb[2] = min(bb[2], f2ord(mx + 4.0f * sx + mg));
+        }
+        if (finitef(my) && finitef(sy)) {
+            bb[1] = max(bb[1], f2ord(my - 4.0f * sy - mg));
+            bb[3] = min(bb[3], f2ord(my + 4.0f * sy + mg));
+        }
+        if (bb[0] > bb[2]) bb[0] = bb[2];
+        if (bb[1] > bb[3]) bb[1] = bb[3];
+    }
+    grids[w] = grid_from_bbox(bb, cell, cell_base[w + 1] - cell_base[w]);
+    wvel[w] = st[2] > 0.0f ? make_float2(st[0] / st[2], st[1] / st[2]) : make_float2(0.0f, 0.0f);
+    uint32_t* nbx = wbbox + ((size_t)(parity ^ 1u) * n_worlds + w) * 4;
+    nbx[0] = 0xFFFFFFFFu;
+    nbx[1] = 0xFFFFFFFFu;
+    nbx[2] = 0u;
+    nbx[3] = 0u;
 }
 
 // Cell key of every snapshot and its rank inside the cell.
@@ -444,6 +525,8 @@ struct SphereScatter {
     uint32_t* bits[3];
     uint32_t oh_size;
     uint32_t nb;
+    float* wstat;
+    uint32_t n_wstat;
 };
 
 // Each snapshot to slot cell_start[key] + rank; the permutation records
@@ -453,6 +536,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t c = i; c <= k.total_cells; c += stride) k.cnt[c] = 0u;
+    for (uint32_t c = i; c < k.n_wstat; c += stride) k.wstat[c] = 0.0f;
     for (uint32_t h = i; h < 3u * k.oh_size; h += stride) {
         const uint32_t t = h / k.oh_size, e = h % k.oh_size;
         k.heads[t][e] = kNil;
@@ -485,8 +569,13 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     if (d.nb == 0) return hipSuccess;
     const uint32_t blocks = div_up(d.nb, kThreads);
+    if (d.n_worlds > 1)
+        k_world_stats<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat);
+    else
+        k_world_stats<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat);
     k_world_grids<<<div_up(d.n_worlds, kThreads), kThreads, 0, d.stream>>>(d.n_worlds, d.wbbox, parity, cell,
-                                                                           d.cell_base, d.grids, d.accum);
+                                                                           d.cell_base, d.wstat, d.grids, d.wvel,
+                                                                           d.accum);
     if (d.n_worlds > 1)
         k_sphere_count<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.wid, d.grids, d.cell_base,
                                                                 d.cell_count, d.keys, d.vals);
@@ -513,6 +602,8 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     }
     k.oh_size = d.oh_size;
     k.nb = d.nb;
+    k.wstat = d.wstat;
+    k.n_wstat = 8u * d.n_worlds;
     k_sphere_scatter<<<blocks, kThreads, 0, d.stream>>>(k);
     e = hipGetLastError();
     // the sorted snapshot becomes the current one; records follow on the next substep
@@ -542,6 +633,7 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.grids = d.grids;
     k.wbbox = d.wbbox + (size_t)a.parity * d.n_worlds * 4;
     k.wold_dt = d.wold_dt;
+    k.wvel = d.wvel;
     const uint32_t j = a.sub;
     k.oh_head = d.oh_head[j % 3];
     k.oh_next = d.oh_next[j % 3];

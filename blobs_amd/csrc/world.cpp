@@ -532,7 +532,7 @@ void World::free_arrays() {
     }
     for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]);
     dfree(d.cell_count), dfree(d.perm);
-    dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt);
+    dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt), dfree(d.wstat), dfree(d.wvel);
     dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
@@ -856,6 +856,10 @@ int World::upload() {
         HIP_TRY(dmalloc(&d.grids, W));
         HIP_TRY(dmalloc(&d.wbbox, 8u * (size_t)W));
         HIP_TRY(dmalloc(&d.wold_dt, W));
+        HIP_TRY(dmalloc(&d.wstat, 8u * (size_t)W));
+        HIP_TRY(dmalloc(&d.wvel, W));
+        HIP_TRY(hipMemsetAsync(d.wstat, 0, sizeof(float) * 8u * W, s));
+        HIP_TRY(hipMemsetAsync(d.wvel, 0, sizeof(float2) * W, s));
         HIP_TRY(h2d(d.cell_base, cell_base, s));
         HIP_TRY(h2d(d.wbbox, wbbox, s));
         HIP_TRY(h2d(d.wold_dt, wold, s));
@@ -1176,6 +1180,8 @@ int World::run_substep(float h, bool fresh) {
     a.has_springs = d.ns > 0 ? 1u : 0u;
     a.skin = skin_;
     a.sub = ++sub_since_rebuild_;
+    a.t_prev = (float)((double)(a.sub - 1) * (double)h);
+    a.t_cur = (float)((double)a.sub * (double)h);
     if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
     if (path_ == StepPath::Sphere) {
         HIP_TRY(launch_sphere_substep(d, a, groups_, sensors_, zero_off_, d.nk > 0));
