@@ -85,7 +85,8 @@ struct DeviceWorld {
     GridParams *grids = nullptr;    // [n_worlds]
     uint32_t *wbbox = nullptr;      // [2][n_worlds][4] f2ord(min x, min y, max x, max y) per parity
     float *wold_dt = nullptr;       // [n_worlds] Physics::old_dt of each world (Q1)
-    float *wstat = nullptr;         // [n_worlds][8] rebuild statistics (sums; see k_world_stats)
+    float *wstat = nullptr;         // [n_worlds][stat_rep][8] rebuild statistics (sums; see k_world_stats)
+    uint32_t stat_rep = 1;          // power of two, stat_rep * n_worlds <= 4096
     float2 *wvel = nullptr;         // [n_worlds] mean velocity of the dynamic bodies at the last rebuild
     uint32_t total_cells = 0;       // cell_base[n_worlds]
     // springs

@@ -402,14 +402,19 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
 
 // Per-world sums over the current snapshot for the next grid and drift:
 // [0,1] velocity of dynamic bodies, [2] their count, [3,4] position,
-// [5,6] squared position, [7] count.  Float atomics: the sums' rounding
+// [5,6] squared position, [7] count.  Each block reduces in LDS and adds
+// into one of `rep` replicas of its world's sums (chosen by block index),
+// so one world's blocks do not all hit the same words.  Float atomics: the sums' rounding
 // order varies, which moves grid bounds and the drift model only -- never
 // a result (any grid and drift give the same pair set).
 template <bool BATCHED>
 __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restrict__ snap,
                                                           const float2* __restrict__ vel,
                                                           const uint32_t* __restrict__ bflags, uint32_t nb,
-                                                          const uint32_t* __restrict__ wid, float* wstat) {
+                                                          const uint32_t* __restrict__ wid, float* wstat,
+                                                          uint32_t rep) {
+    __shared__ float s_part[kThreads / BG_WAVE][8];
+    __shared__ uint32_t s_w[kThreads / BG_WAVE];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < nb;
     const uint32_t w = BATCHED ? wid[live ? i : nb - 1] : 0u;
@@ -434,16 +439,41 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
             v[7] = 1.0f;
         }
     }
+    const int lane = threadIdx.x & (BG_WAVE - 1), wv = threadIdx.x / BG_WAVE;
     const uint32_t w0 = __shfl(w, 0, BG_WAVE);
-    float* dst = wstat + (size_t)w * 8;
-    if (__all(w == w0)) {
+    const bool wave_uniform = __all(w == w0);
+    float* dst = wstat + ((size_t)w * rep + (blockIdx.x & (rep - 1))) * 8;
+    if (!wave_uniform) {  // a wave across worlds: per-lane atomics
+        if (v[7] > 0.0f)
+            for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
+    } else {
 #pragma unroll
         for (int t = 0; t < 8; ++t)
             for (int o = 32; o > 0; o >>= 1) v[t] += __shfl_xor(v[t], o, BG_WAVE);
-        if ((threadIdx.x & (BG_WAVE - 1)) == 0 && v[7] > 0.0f)
-            for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
-    } else if (v[7] > 0.0f) {
-        for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
+    }
+    if (lane == 0) {
+        s_w[wv] = wave_uniform ? w : kNil;
+        for (int t = 0; t < 8; ++t) s_part[wv][t] = v[t];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // one atomic set for the waves that share the first wave's world, one per other wave
+        float acc[8];
+        for (int t = 0; t < 8; ++t) acc[t] = 0.0f;
+        const uint32_t wb = s_w[0];
+        for (int k = 0; k < kThreads / BG_WAVE; ++k) {
+            if (s_w[k] == kNil) continue;
+            if (s_w[k] == wb) {
+                for (int t = 0; t < 8; ++t) acc[t] += s_part[k][t];
+            } else if (s_part[k][7] > 0.0f) {
+                float* d2 = wstat + ((size_t)s_w[k] * rep + (blockIdx.x & (rep - 1))) * 8;
+                for (int t = 0; t < 8; ++t) atomicAdd(&d2[t], s_part[k][t]);
+            }
+        }
+        if (wb != kNil && acc[7] > 0.0f) {
+            float* d2 = wstat + ((size_t)wb * rep + (blockIdx.x & (rep - 1))) * 8;
+            for (int t = 0; t < 8; ++t) atomicAdd(&d2[t], acc[t]);
+        }
     }
 }
 
@@ -452,17 +482,25 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
 // few far-flung bodies (clamped into border cells) do not coarsen the cells
 // of the rest; the mean velocity of its dynamic bodies (drift model); the
 // other parity's boxes restart empty.
-__global__ __launch_bounds__(kThreads) void k_world_grids(uint32_t n_worlds, uint32_t* wbbox, uint32_t parity,
-                                                          float cell, const uint32_t* __restrict__ cell_base,
-                                                          const float* __restrict__ wstat, GridParams* grids,
-                                                          float2* wvel, StepAccum* accum) {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w == 0) {
+// One wave per world (the replica sums are reduced across its lanes).
+__global__ __launch_bounds__(BG_WAVE) void k_world_grids(uint32_t n_worlds, uint32_t* wbbox, uint32_t parity,
+                                                         float cell, const uint32_t* __restrict__ cell_base,
+                                                         const float* __restrict__ wstat, uint32_t rep,
+                                                         GridParams* grids, float2* wvel, StepAccum* accum) {
+    const uint32_t w = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    if (w == 0 && lane == 0) {
         accum->dmax = 0u;
         accum->n_outliers = 0u;
     }
     if (w >= n_worlds) return;
-    const float* st = wstat + (size_t)w * 8;
+    float st[8];
+    for (int t = 0; t < 8; ++t) st[t] = 0.0f;
+    for (uint32_t r = lane; r < rep; r += BG_WAVE)
+        for (int t = 0; t < 8; ++t) st[t] += wstat[((size_t)w * rep + r) * 8 + t];
+    for (int t = 0; t < 8; ++t)
+        for (int o = 32; o > 0; o >>= 1) st[t] += __shfl_xor(st[t], o, BG_WAVE);
+    if (lane != 0) return;
     uint32_t bb[4];
     const uint32_t* src = wbbox + ((size_t)parity * n_worlds + w) * 4;
     for (int t = 0; t < 4; ++t) bb[t] = src[t];
@@ -570,12 +608,14 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     if (d.nb == 0) return hipSuccess;
     const uint32_t blocks = div_up(d.nb, kThreads);
     if (d.n_worlds > 1)
-        k_world_stats<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat);
+        k_world_stats<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
+                                                               d.stat_rep);
     else
-        k_world_stats<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat);
-    k_world_grids<<<div_up(d.n_worlds, kThreads), kThreads, 0, d.stream>>>(d.n_worlds, d.wbbox, parity, cell,
-                                                                           d.cell_base, d.wstat, d.grids, d.wvel,
-                                                                           d.accum);
+        k_world_stats<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
+                                                                d.stat_rep);
+    k_world_grids<<<d.n_worlds, BG_WAVE, 0, d.stream>>>(d.n_worlds, d.wbbox, parity, cell,
+                                                                           d.cell_base, d.wstat, d.stat_rep,
+                                                                           d.grids, d.wvel, d.accum);
     if (d.n_worlds > 1)
         k_sphere_count<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.wid, d.grids, d.cell_base,
                                                                 d.cell_count, d.keys, d.vals);
@@ -603,7 +643,7 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     k.oh_size = d.oh_size;
     k.nb = d.nb;
     k.wstat = d.wstat;
-    k.n_wstat = 8u * d.n_worlds;
+    k.n_wstat = 8u * d.stat_rep * d.n_worlds;
     k_sphere_scatter<<<blocks, kThreads, 0, d.stream>>>(k);
     e = hipGetLastError();
     // the sorted snapshot becomes the current one; records follow on the next substep

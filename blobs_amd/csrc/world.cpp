@@ -856,9 +856,11 @@ int World::upload() {
         HIP_TRY(dmalloc(&d.grids, W));
         HIP_TRY(dmalloc(&d.wbbox, 8u * (size_t)W));
         HIP_TRY(dmalloc(&d.wold_dt, W));
-        HIP_TRY(dmalloc(&d.wstat, 8u * (size_t)W));
+        d.stat_rep = 1;
+        while (d.stat_rep < 256u && (size_t)d.stat_rep * 2u * W <= 4096u) d.stat_rep *= 2u;
+        HIP_TRY(dmalloc(&d.wstat, 8u * d.stat_rep * (size_t)W));
         HIP_TRY(dmalloc(&d.wvel, W));
-        HIP_TRY(hipMemsetAsync(d.wstat, 0, sizeof(float) * 8u * W, s));
+        HIP_TRY(hipMemsetAsync(d.wstat, 0, sizeof(float) * 8u * d.stat_rep * W, s));
         HIP_TRY(hipMemsetAsync(d.wvel, 0, sizeof(float2) * W, s));
         HIP_TRY(h2d(d.cell_base, cell_base, s));
         HIP_TRY(h2d(d.wbbox, wbbox, s));
