@@ -83,7 +83,7 @@ struct DeviceWorld {
     uint32_t *wid = nullptr;        // world of each slot (batch only)
     uint32_t *cell_base = nullptr;  // [n_worlds + 1]
     GridParams *grids = nullptr;    // [n_worlds]
-    uint32_t *wbbox = nullptr;      // [2][n_worlds][4] f2ord(min x, min y, max x, max y) per parity
+    uint32_t *wbbox = nullptr;      // [n_worlds][4] f2ord(min x, min y, max x, max y) of the snapshot being sorted
     float *wold_dt = nullptr;       // [n_worlds] Physics::old_dt of each world (Q1)
     float *wstat = nullptr;         // [n_worlds][stat_rep][8] rebuild statistics (sums; see k_world_stats)
     uint32_t stat_rep = 1;          // power of two, stat_rep * n_worlds <= 4096

@@ -49,7 +49,6 @@ struct SphereSubstep {
     const uint32_t* __restrict__ wid;
     const uint32_t* __restrict__ cell_base;
     const GridParams* __restrict__ grids;
-    uint32_t* wbbox;  // this substep's parity: [n_worlds][4]
     float* wold_dt;
     const float2* __restrict__ wvel;
     const uint32_t* __restrict__ oh_head;  // outlier table read this substep
@@ -107,32 +106,8 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
     }
 }
 
-// Bounding box of this substep's snapshots per world: one conditional atomic
-// set per wave when the wave lies in one world, per lane otherwise.
-__device__ __forceinline__ void flush_world_bbox(const LocalAccum& la, uint32_t w, uint32_t* wbbox) {
-    const uint32_t w0 = __shfl(w, 0, BG_WAVE);
-    const int lane = threadIdx.x & (BG_WAVE - 1);
-    uint32_t v0 = f2ord(la.minx), v1 = f2ord(la.miny), v2 = f2ord(la.maxx), v3 = f2ord(la.maxy);
-    if (__all(w == w0)) {
-        v0 = wave_min_u32(v0);
-        v1 = wave_min_u32(v1);
-        v2 = wave_max_u32(v2);
-        v3 = wave_max_u32(v3);
-        if (lane != 0 || v0 > v2) return;  // nothing finite in the wave
-    } else if (v0 > v2) {
-        return;
-    }
-    uint32_t* b = wbbox + (size_t)w * 4;
-    atomic_min_if_less(&b[0], v0);
-    atomic_min_if_less(&b[1], v1);
-    atomic_max_if_greater(&b[2], v2);
-    atomic_max_if_greater(&b[3], v3);
-}
-
 template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER, bool BATCHED>
 __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
-    LocalAccum la;
-    la.init();
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kThreads + threadIdx.x;
     // clear the outlier table of the substep after next
@@ -142,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
     }
     // world of this slot (lanes past the end take the last body's world)
     const uint32_t w = BATCHED ? k.wid[s < k.nb ? s : k.nb - 1] : 0u;
-    uint32_t new_outliers = 0;
+    uint32_t new_outliers = 0, count = 0, pushes = 0, flags = 0;
     if (s < k.nb) {
         const uint32_t g_own = GATHER ? k.perm[s] : s;
         const uint32_t fl = k.src.bflags[g_own];
@@ -156,7 +131,6 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         const bool was_out = (wa & kSnapOutlier) != 0u;
         const float2 mgv = k.src.mg[g_own];
         const float mown = mgv.x;
-        uint32_t count = 0, pushes = 0, flags = 0;
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
             const float ra = own.z;
@@ -176,16 +150,17 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                     const uint2 gb = k.src.groups[GATHER ? k.perm[t] : t];
                     if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) return;
                 }
-                const uint32_t idb = __float_as_uint(sb.w) & ~kSnapOutlier;
-                const bool isA = id > idb;
-                const float ax = isA ? xa.x - sb.x : sb.x - xa.x;
-                const float ay = isA ? xa.y - sb.y : sb.y - xa.y;
+                // axis = A - B with A the higher collider index; b - a == -(a - b) exactly,
+                // so the unoriented difference decides and is flipped only for a hit
+                const float dx = xa.x - sb.x, dy = xa.y - sb.y;
                 const float rs = ra + sb.z;
-                const float dot = ax * ax + ay * ay;
+                const float dot = dx * dx + dy * dy;
                 if (dot > (rs * rs) * 1.00000095367431640625f) return;
                 const float dist = sqrtf(dot);
                 if (!(dist < rs)) return;
-                sink(t, idb, isA, ax, ay, dist, rs);
+                const uint32_t idb = __float_as_uint(sb.w) & ~kSnapOutlier;
+                const bool isA = id > idb;
+                sink(t, idb, isA, isA ? dx : -dx, isA ? dy : -dy, dist, rs);
             };
             auto enumerate = [&](auto&& sink) {
                 // sorted grid: four candidates per load batch (clamped indices, no branches)
@@ -365,7 +340,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             const float2 vw = k.wvel[w];
             const float dx = a.drift_x + vw.x * a.t_cur, dy = a.drift_y + vw.y * a.t_cur;
             const float ex = (x.x - x0.x) - dx, ey = (x.y - x0.y) - dy;
-            out = !(sqrtf(ex * ex + ey * ey) <= a.skin);
+            out = !(ex * ex + ey * ey <= a.skin * a.skin);
             new_outliers += out ? 1u : 0u;
         }
         k.snap_out[s] = make_float4(x.x, x.y, own.z, __uint_as_float(id | (out ? kSnapOutlier : 0u)));
@@ -378,23 +353,18 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             k.oh_ins_next[s] = old;
             atomicOr(&k.oh_ins_bits[h >> 5], 1u << (h & 31u));
         }
-        la.point(x.x, x.y, x.x, x.y, 0.0f, 0.0f);
         if (CONSTRAINTS) apply_constraints(st.p, k.con_pos, k.con_r, k.nk);
         k.dst.pp[s] = make_float4(st.p.x, st.p.y, st.po.x, st.po.y);
-        la.contacts = count;
-        la.push_outs = pushes;
-        la.flags = flags;
     }
-    flush_world_bbox(la, w, k.wbbox);
-    // outlier count: one atomic per block that produced any
-    const uint32_t no = wave_sum_u32(new_outliers);
-    __shared__ uint32_t s_no[kThreads / BG_WAVE];
-    if ((threadIdx.x & (BG_WAVE - 1)) == 0) s_no[threadIdx.x / BG_WAVE] = no;
-    flush_accum(la, k.accum, k.block_contacts, a.parity, false);
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int i = 0; i < kThreads / BG_WAVE; ++i) tot += s_no[i];
-        if (tot) atomicAdd(&k.accum->n_outliers, tot);
+    // counters: two wave sums, then one atomic per wave and quantity that is non-zero
+    // (contacts into this block's slot, summed on the host when queried)
+    if (flags) atomicOr(&k.accum->error_flags, flags);
+    const uint32_t c_sum = wave_sum_u32(count);
+    const uint32_t po = wave_sum_u32((pushes << 16) | new_outliers);
+    if ((threadIdx.x & (BG_WAVE - 1)) == 0) {
+        if (c_sum) atomicAdd(&k.block_contacts[blockIdx.x], (unsigned long long)c_sum);
+        if (po >> 16) atomicAdd(&k.accum->push_outs, po >> 16);
+        if (po & 0xFFFFu) atomicAdd(&k.accum->n_outliers, po & 0xFFFFu);
     }
 }
 
@@ -402,7 +372,8 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
 
 // Per-world sums over the current snapshot for the next grid and drift:
 // [0,1] velocity of dynamic bodies, [2] their count, [3,4] position,
-// [5,6] squared position, [7] count.  Each block reduces in LDS and adds
+// [5,6] squared position, [7] count, and the bounding box (ordered-int
+// atomics).  Each block reduces in LDS and adds
 // into one of `rep` replicas of its world's sums (chosen by block index),
 // so one world's blocks do not all hit the same words.  Float atomics: the sums' rounding
 // order varies, which moves grid bounds and the drift model only -- never
@@ -412,8 +383,9 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
                                                           const float2* __restrict__ vel,
                                                           const uint32_t* __restrict__ bflags, uint32_t nb,
                                                           const uint32_t* __restrict__ wid, float* wstat,
-                                                          uint32_t rep) {
+                                                          uint32_t rep, uint32_t* wbbox) {
     __shared__ float s_part[kThreads / BG_WAVE][8];
+    __shared__ uint32_t s_bb[kThreads / BG_WAVE][4];
     __shared__ uint32_t s_w[kThreads / BG_WAVE];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < nb;
@@ -421,9 +393,12 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
     float v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = 0.0f;
+    uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0u, b3 = 0u;  // f2ord bounding box
     if (live) {
         const float4 sn = snap[i];
         if (finitef(sn.x) && finitef(sn.y)) {
+            b0 = b2 = f2ord(sn.x);
+            b1 = b3 = f2ord(sn.y);
             if (!(bflags[i] & BF_STATIC)) {
                 const float2 vv = vel[i];
                 if (finitef(vv.x) && finitef(vv.y)) {
@@ -444,47 +419,74 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
     const bool wave_uniform = __all(w == w0);
     float* dst = wstat + ((size_t)w * rep + (blockIdx.x & (rep - 1))) * 8;
     if (!wave_uniform) {  // a wave across worlds: per-lane atomics
-        if (v[7] > 0.0f)
+        if (v[7] > 0.0f) {
             for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
+            uint32_t* bb = wbbox + ((size_t)w * rep + (blockIdx.x & (rep - 1))) * 4;
+            atomic_min_if_less(&bb[0], b0);
+            atomic_min_if_less(&bb[1], b1);
+            atomic_max_if_greater(&bb[2], b2);
+            atomic_max_if_greater(&bb[3], b3);
+        }
     } else {
 #pragma unroll
         for (int t = 0; t < 8; ++t)
             for (int o = 32; o > 0; o >>= 1) v[t] += __shfl_xor(v[t], o, BG_WAVE);
+        b0 = wave_min_u32(b0);
+        b1 = wave_min_u32(b1);
+        b2 = wave_max_u32(b2);
+        b3 = wave_max_u32(b3);
     }
     if (lane == 0) {
         s_w[wv] = wave_uniform ? w : kNil;
         for (int t = 0; t < 8; ++t) s_part[wv][t] = v[t];
+        s_bb[wv][0] = b0;
+        s_bb[wv][1] = b1;
+        s_bb[wv][2] = b2;
+        s_bb[wv][3] = b3;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         // one atomic set for the waves that share the first wave's world, one per other wave
         float acc[8];
         for (int t = 0; t < 8; ++t) acc[t] = 0.0f;
+        uint32_t ab[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
         const uint32_t wb = s_w[0];
         for (int k = 0; k < kThreads / BG_WAVE; ++k) {
             if (s_w[k] == kNil) continue;
             if (s_w[k] == wb) {
                 for (int t = 0; t < 8; ++t) acc[t] += s_part[k][t];
+                ab[0] = min(ab[0], s_bb[k][0]);
+                ab[1] = min(ab[1], s_bb[k][1]);
+                ab[2] = max(ab[2], s_bb[k][2]);
+                ab[3] = max(ab[3], s_bb[k][3]);
             } else if (s_part[k][7] > 0.0f) {
                 float* d2 = wstat + ((size_t)s_w[k] * rep + (blockIdx.x & (rep - 1))) * 8;
                 for (int t = 0; t < 8; ++t) atomicAdd(&d2[t], s_part[k][t]);
+                uint32_t* bb = wbbox + ((size_t)s_w[k] * rep + (blockIdx.x & (rep - 1))) * 4;
+                atomic_min_if_less(&bb[0], s_bb[k][0]);
+                atomic_min_if_less(&bb[1], s_bb[k][1]);
+                atomic_max_if_greater(&bb[2], s_bb[k][2]);
+                atomic_max_if_greater(&bb[3], s_bb[k][3]);
             }
         }
         if (wb != kNil && acc[7] > 0.0f) {
             float* d2 = wstat + ((size_t)wb * rep + (blockIdx.x & (rep - 1))) * 8;
             for (int t = 0; t < 8; ++t) atomicAdd(&d2[t], acc[t]);
+            uint32_t* bb = wbbox + ((size_t)wb * rep + (blockIdx.x & (rep - 1))) * 4;
+            atomic_min_if_less(&bb[0], ab[0]);
+            atomic_min_if_less(&bb[1], ab[1]);
+            atomic_max_if_greater(&bb[2], ab[2]);
+            atomic_max_if_greater(&bb[3], ab[3]);
         }
     }
 }
 
-// Grid of every world over its snapshots: the bounding box accumulated
-// over the period that ends here, clipped to the centroid +- 4 sigma so a
-// few far-flung bodies (clamped into border cells) do not coarsen the cells
-// of the rest; the mean velocity of its dynamic bodies (drift model); the
-// other parity's boxes restart empty.
+// Grid of every world over its snapshots: their bounding box clipped to the
+// centroid +- 4 sigma so a few far-flung bodies (clamped into border cells)
+// do not coarsen the cells of the rest; the mean velocity of its dynamic
+// bodies (drift model).
 // One wave per world (the replica sums are reduced across its lanes).
-__global__ __launch_bounds__(BG_WAVE) void k_world_grids(uint32_t n_worlds, uint32_t* wbbox, uint32_t parity,
-                                                         float cell, const uint32_t* __restrict__ cell_base,
+__global__ __launch_bounds__(BG_WAVE) void k_world_grids(uint32_t n_worlds, uint32_t* wbbox, float cell, const uint32_t* __restrict__ cell_base,
                                                          const float* __restrict__ wstat, uint32_t rep,
                                                          GridParams* grids, float2* wvel, StepAccum* accum) {
     const uint32_t w = blockIdx.x;
@@ -496,14 +498,24 @@ __global__ __launch_bounds__(BG_WAVE) void k_world_grids(uint32_t n_worlds, uint
     if (w >= n_worlds) return;
     float st[8];
     for (int t = 0; t < 8; ++t) st[t] = 0.0f;
-    for (uint32_t r = lane; r < rep; r += BG_WAVE)
+    uint32_t bb[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
+    for (uint32_t r = lane; r < rep; r += BG_WAVE) {
         for (int t = 0; t < 8; ++t) st[t] += wstat[((size_t)w * rep + r) * 8 + t];
+        uint32_t* src = wbbox + ((size_t)w * rep + r) * 4;
+        bb[0] = min(bb[0], src[0]);
+        bb[1] = min(bb[1], src[1]);
+        bb[2] = max(bb[2], src[2]);
+        bb[3] = max(bb[3], src[3]);
+        src[0] = src[1] = 0xFFFFFFFFu;  // empty again for the next rebuild's statistics
+        src[2] = src[3] = 0u;
+    }
     for (int t = 0; t < 8; ++t)
         for (int o = 32; o > 0; o >>= 1) st[t] += __shfl_xor(st[t], o, BG_WAVE);
+    bb[0] = wave_min_u32(bb[0]);
+    bb[1] = wave_min_u32(bb[1]);
+    bb[2] = wave_max_u32(bb[2]);
+    bb[3] = wave_max_u32(bb[3]);
     if (lane != 0) return;
-    uint32_t bb[4];
-    const uint32_t* src = wbbox + ((size_t)parity * n_worlds + w) * 4;
-    for (int t = 0; t < 4; ++t) bb[t] = src[t];
     if (st[7] > 0.0f && bb[0] <= bb[2] && bb[1] <= bb[3]) {
         const float n = st[7];
         const float mx = st[3] / n, my = st[4] / n;
@@ -522,11 +534,6 @@ __global__ __launch_bounds__(BG_WAVE) void k_world_grids(uint32_t n_worlds, uint
     }
     grids[w] = grid_from_bbox(bb, cell, cell_base[w + 1] - cell_base[w]);
     wvel[w] = st[2] > 0.0f ? make_float2(st[0] / st[2], st[1] / st[2]) : make_float2(0.0f, 0.0f);
-    uint32_t* nbx = wbbox + ((size_t)(parity ^ 1u) * n_worlds + w) * 4;
-    nbx[0] = 0xFFFFFFFFu;
-    nbx[1] = 0xFFFFFFFFu;
-    nbx[2] = 0u;
-    nbx[3] = 0u;
 }
 
 // Cell key of every snapshot and its rank inside the cell.
@@ -609,11 +616,11 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     const uint32_t blocks = div_up(d.nb, kThreads);
     if (d.n_worlds > 1)
         k_world_stats<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
-                                                               d.stat_rep);
+                                                               d.stat_rep, d.wbbox);
     else
         k_world_stats<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
-                                                                d.stat_rep);
-    k_world_grids<<<d.n_worlds, BG_WAVE, 0, d.stream>>>(d.n_worlds, d.wbbox, parity, cell,
+                                                                d.stat_rep, d.wbbox);
+    k_world_grids<<<d.n_worlds, BG_WAVE, 0, d.stream>>>(d.n_worlds, d.wbbox, cell,
                                                                            d.cell_base, d.wstat, d.stat_rep,
                                                                            d.grids, d.wvel, d.accum);
     if (d.n_worlds > 1)
@@ -671,7 +678,6 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.wid = d.wid;
     k.cell_base = d.cell_base;
     k.grids = d.grids;
-    k.wbbox = d.wbbox + (size_t)a.parity * d.n_worlds * 4;
     k.wold_dt = d.wold_dt;
     k.wvel = d.wvel;
     const uint32_t j = a.sub;

@@ -815,7 +815,6 @@ int World::upload() {
         std::vector<uint32_t> cell_base(W + 1, 0), wid;
         std::vector<float> wold(W, params_.old_dt);
         if (!batch_old_dt_.empty()) wold = batch_old_dt_;
-        std::vector<uint32_t> wbbox(8u * W, 0u);
         if (W > 1) wid.resize(nb);
         for (uint32_t w = 0; w < W; ++w) {
             const uint32_t n_w = wstart[w + 1] - wstart[w];
@@ -823,7 +822,6 @@ int World::upload() {
             const uint64_t base = (uint64_t)cell_base[w] + ((cap + 63ull) & ~63ull);
             if (base >= 0xFFFFFFF0ull) return fail(BG_ERR_CAPACITY, "cell table of the batch exceeds 2^32 entries");
             cell_base[w + 1] = (uint32_t)base;
-            float bx0 = INFINITY, by0 = INFINITY, bx1 = -INFINITY, by1 = -INFINITY;
             bool first = true;
             for (uint32_t i = wstart[w]; i < wstart[w + 1]; ++i) {
                 if (W > 1) wid[i] = w;
@@ -831,22 +829,7 @@ int World::upload() {
                     bflags[i] |= BF_FIRST_DYN;
                     first = false;
                 }
-                const float x = cabs[i].x, y = cabs[i].y;
-                if (isfinite(x) && isfinite(y)) {
-                    bx0 = fminf(bx0, x);
-                    bx1 = fmaxf(bx1, x);
-                    by0 = fminf(by0, y);
-                    by1 = fmaxf(by1, y);
-                }
             }
-            uint32_t* b0 = &wbbox[(size_t)w * 4];
-            b0[0] = f2ord_host(bx0);
-            b0[1] = f2ord_host(by0);
-            b0[2] = f2ord_host(bx1);
-            b0[3] = f2ord_host(by1);
-            uint32_t* b1 = &wbbox[((size_t)W + w) * 4];
-            b1[0] = b1[1] = 0xFFFFFFFFu;
-            b1[2] = b1[3] = 0u;
         }
         if (W == 1 && cell_base[1] < d.cells_cap) cell_base[1] = d.cells_cap;
         d.n_worlds = W;
@@ -854,16 +837,21 @@ int World::upload() {
         cell_rows = d.total_cells;
         HIP_TRY(dmalloc(&d.cell_base, W + 1));
         HIP_TRY(dmalloc(&d.grids, W));
-        HIP_TRY(dmalloc(&d.wbbox, 8u * (size_t)W));
-        HIP_TRY(dmalloc(&d.wold_dt, W));
         d.stat_rep = 1;
         while (d.stat_rep < 256u && (size_t)d.stat_rep * 2u * W <= 4096u) d.stat_rep *= 2u;
+        HIP_TRY(dmalloc(&d.wbbox, 4u * d.stat_rep * (size_t)W));
+        HIP_TRY(dmalloc(&d.wold_dt, W));
         HIP_TRY(dmalloc(&d.wstat, 8u * d.stat_rep * (size_t)W));
         HIP_TRY(dmalloc(&d.wvel, W));
         HIP_TRY(hipMemsetAsync(d.wstat, 0, sizeof(float) * 8u * d.stat_rep * W, s));
         HIP_TRY(hipMemsetAsync(d.wvel, 0, sizeof(float2) * W, s));
         HIP_TRY(h2d(d.cell_base, cell_base, s));
-        HIP_TRY(h2d(d.wbbox, wbbox, s));
+        {
+            std::vector<uint32_t> wbbox(4u * d.stat_rep * (size_t)W);  // empty boxes, filled by each rebuild
+            for (size_t i = 0; i < wbbox.size(); ++i) wbbox[i] = (i & 2u) ? 0u : 0xFFFFFFFFu;
+            HIP_TRY(h2d(d.wbbox, wbbox, s));
+            HIP_TRY(hipStreamSynchronize(s));  // the host vector dies here
+        }
         HIP_TRY(h2d(d.wold_dt, wold, s));
         if (W > 1) {
             HIP_TRY(dmalloc(&d.wid, nb));
