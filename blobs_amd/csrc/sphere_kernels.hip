@@ -37,6 +37,7 @@ namespace bg {
 namespace {
 
 constexpr int kCoarseShift = 2;  // outlier table cells are 4x4 grid cells
+constexpr int kFlatRows = 4;      // grid rows a query walks as one flat candidate list (<= 64 candidates)
 
 struct SphereSubstep {
     SphereArrays src;  // body records read (through perm on the first substep after a rebuild)
@@ -81,22 +82,30 @@ __device__ __forceinline__ uint32_t cell_hash(int32_t cx, int32_t cy, uint32_t w
 // Rows of the sorted grid overlapping the square of half-width Q around
 // (qx, qy): calls row(t0, t1) with the slot range of each row.  Cell
 // coordinates are monotone in x, so the rows cover every x0 within Q.
-template <class F>
-__device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* __restrict__ cell_start, uint32_t n,
-                                          float qx, float qy, float Q, F&& row) {
+// Cell rectangle of the square of half-width Q around (qx, qy), padded for
+// rounding; false (and the whole grid) when the query is not finite.
+__device__ __forceinline__ bool query_rect(const GridParams& g, float qx, float qy, float Q, int32_t& cx0,
+                                           int32_t& cx1, int32_t& cy0, int32_t& cy1) {
     const float R = Q + (1e-5f * (fabsf(qx) + fabsf(qy) + Q) + 1e-6f);
-    int32_t cx0, cx1, cy0, cy1;
     if (R < 1e30f && finitef(qx) && finitef(qy)) {
         cx0 = cell_coord(qx - R, g.ox, g.inv_cell, g.nx);
         cx1 = cell_coord(qx + R, g.ox, g.inv_cell, g.nx);
         cy0 = cell_coord(qy - R, g.oy, g.inv_cell, g.ny);
         cy1 = cell_coord(qy + R, g.oy, g.inv_cell, g.ny);
-    } else {
-        cx0 = 0;
-        cx1 = g.nx - 1;
-        cy0 = 0;
-        cy1 = g.ny - 1;
+        return true;
     }
+    cx0 = 0;
+    cx1 = g.nx - 1;
+    cy0 = 0;
+    cy1 = g.ny - 1;
+    return false;
+}
+
+template <class F>
+__device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* __restrict__ cell_start, uint32_t n,
+                                          float qx, float qy, float Q, F&& row) {
+    int32_t cx0, cx1, cy0, cy1;
+    query_rect(g, qx, qy, Q, cx0, cx1, cy0, cy1);
     for (int32_t cy = cy0; cy <= cy1; ++cy) {
         const uint32_t rowc = (uint32_t)cy * (uint32_t)g.nx;
         const uint32_t t0 = cell_start[rowc + (uint32_t)cx0];
@@ -162,22 +171,76 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                 const bool isA = id > idb;
                 sink(t, idb, isA, isA ? dx : -dx, isA ? dy : -dy, dist, rs);
             };
+            // Sorted-grid candidates.  With at most kFlatRows rows and 64 candidates (the
+            // common case) they form one flat list: a cheap pass marks the candidates within
+            // the squared contact distance in a 64-bit mask, and only marked ones take the
+            // exact test and the correction -- so a wave pays the correction path once per
+            // hit of its busiest lane, not once per candidate.
+            int32_t qcx0, qcx1, qcy0, qcy1;
+            const bool rect_ok = query_rect(g, xa.x - dpx, xa.y - dpy, ra + a.rmax + a.skin, qcx0, qcx1, qcy0, qcy1);
+            uint32_t rlo[kFlatRows], rend[kFlatRows];  // slot range start, flat end of each row
+            uint32_t n_flat = 0;
+            bool flat = rect_ok && qcy1 - qcy0 < kFlatRows;
+            if (flat) {
+#pragma unroll
+                for (int r = 0; r < kFlatRows; ++r) {
+                    const int32_t cy = qcy0 + r;
+                    uint32_t lo = 0, hi = 0;
+                    if (cy <= qcy1) {
+                        const uint32_t rowc = (uint32_t)cy * (uint32_t)g.nx;
+                        lo = cs[rowc + (uint32_t)qcx0];
+                        hi = cs[rowc + (uint32_t)qcx1 + 1u];
+                        hi = hi < k.nb ? hi : k.nb;
+                        hi = hi > lo ? hi : lo;
+                    }
+                    rlo[r] = lo;
+                    n_flat += hi - lo;
+                    rend[r] = n_flat;
+                }
+                flat = n_flat <= 64u;
+            }
+            auto flat_slot = [&](uint32_t i) {
+                uint32_t t = rlo[0] + i;
+#pragma unroll
+                for (int r = 1; r < kFlatRows; ++r)
+                    if (i >= rend[r - 1]) t = rlo[r] + (i - rend[r - 1]);
+                return t;
+            };
+            uint64_t near = 0;  // flat candidates within the squared contact distance
+            if (flat) {
+                for (uint32_t f = 0; f < n_flat; f += 4) {
+                    uint32_t tt[4];
+                    float4 sb[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        tt[u] = flat_slot(min(f + (uint32_t)u, n_flat - 1));
+                        sb[u] = k.snap_in[tt[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float dx = xa.x - sb[u].x, dy = xa.y - sb[u].y;
+                        const float rs = ra + sb[u].z;
+                        const bool cand = f + u < n_flat && tt[u] != s && !(__float_as_uint(sb[u].w) & kSnapOutlier) &&
+                                          !(dx * dx + dy * dy > (rs * rs) * 1.00000095367431640625f);
+                        near |= cand ? (1ull << (f + u)) : 0ull;
+                    }
+                }
+            }
             auto enumerate = [&](auto&& sink) {
-                // sorted grid: four candidates per load batch (clamped indices, no branches)
-                grid_rows(g, cs, k.nb, xa.x - dpx, xa.y - dpy, ra + a.rmax + a.skin,
-                          [&](uint32_t t0, uint32_t t1) {
-                              for (uint32_t t = t0; t < t1; t += 4) {
-                                  float4 sb[4];
-#pragma unroll
-                                  for (int u = 0; u < 4; ++u) sb[u] = k.snap_in[min(t + u, t1 - 1)];
-#pragma unroll
-                                  for (int u = 0; u < 4; ++u) {
-                                      const uint32_t tu = t + u;
-                                      if (tu < t1 && tu != s && !(__float_as_uint(sb[u].w) & kSnapOutlier))
-                                          test(tu, sb[u], sink);
+                if (flat) {
+                    for (uint64_t m = near; m != 0; m &= m - 1) {
+                        const uint32_t t = flat_slot((uint32_t)__builtin_ctzll(m));
+                        test(t, k.snap_in[t], sink);
+                    }
+                } else {
+                    grid_rows(g, cs, k.nb, xa.x - dpx, xa.y - dpy, ra + a.rmax + a.skin,
+                              [&](uint32_t t0, uint32_t t1) {
+                                  for (uint32_t t = t0; t < t1; ++t) {
+                                      const float4 sb = k.snap_in[t];
+                                      if (t != s && !(__float_as_uint(sb.w) & kSnapOutlier)) test(t, sb, sink);
                                   }
-                              }
-                          });
+                              });
+                }
                 if (nout == 0u) return;
                 // outlier table: coarse cells of this world around the current snapshot
                 float R = ra + a.rmax;
