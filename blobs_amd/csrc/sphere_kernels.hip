@@ -139,6 +139,9 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         const uint32_t id = wa & ~kSnapOutlier;
         const bool was_out = (wa & kSnapOutlier) != 0u;
         const float2 mgv = k.src.mg[g_own];
+        // loaded now, used after the contact pass: their latency overlaps the candidate walk
+        const float2 rw = k.src.rw[g_own];
+        const float2 x0 = k.x0[s];
         const float mown = mgv.x;
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
@@ -350,7 +353,6 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             }
         }
         // update_objects (physics.rs:326-358)
-        const float2 rw = k.src.rw[g_own];
         st.rot = rw.x;
         st.omega = rw.y;
         float2 vel;
@@ -399,7 +401,6 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
         const bool finite_x = finitef(x.x) && finitef(x.y);
         bool out = was_out;
         if (a.collisions && !out && finite_x) {
-            const float2 x0 = k.x0[s];
             const float2 vw = k.wvel[w];
             const float dx = a.drift_x + vw.x * a.t_cur, dy = a.drift_y + vw.y * a.t_cur;
             const float ex = (x.x - x0.x) - dx, ey = (x.y - x0.y) - dy;
