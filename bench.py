@@ -107,7 +107,7 @@ def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) 
     tuned = None
     if autotune:
         t0 = time.perf_counter()
-        cs, sk, iv = batch.autotune(dt, 2)
+        cs, sk, iv = batch.autotune(dt, 4)
         tuned = {"cell_scale": cs, "skin": sk, "rebuild_interval": iv}
         log(f"batched: autotune -> {tuned} ({time.perf_counter() - t0:.1f} s)")
     batch.run_chunks(dt, chunk_steps, 1)  # graph capture warm-up
@@ -205,11 +205,16 @@ def main() -> int:
     if args.rebuild_interval or args.cell_scale or args.skin:
         p.set_broadphase(args.cell_scale or 1.0, args.rebuild_interval or 1, args.skin or 0.5)
 
-    # warmup (uploads the world, builds the grid)
+    # warmup (uploads the world, builds the grid); the autotuner picks the
+    # broadphase settings over real steps (results do not depend on them)
     t0 = time.perf_counter()
     p.step(sc.dt, args.warmup)
+    tuned = None
+    if not (args.no_autotune or args.rebuild_interval or args.cell_scale or args.skin):
+        cs_, sk_, iv_ = p.autotune(sc.dt, 4)
+        tuned = {"cell_scale": cs_, "skin": sk_, "rebuild_interval": iv_}
     p._lib.bg_synchronize(p._w)
-    log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, path {p.step_path()}")
+    log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, path {p.step_path()}, autotuned {tuned}")
 
     p.set_profiling(True)
     p._lib.bg_synchronize(p._w)
@@ -262,6 +267,7 @@ def main() -> int:
             "dt": sc.dt,
             "substeps": 8,
             "path": p.step_path(),
+            "broadphase_autotuned": tuned,
         },
         "roofline": {
             "bound": "hbm",
