@@ -70,26 +70,42 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tile_sums(uint32_t* sums,
     }
 }
 
-// out = exclusive scan of in; thread t owns items [t*16, t*16+16) of its tile.
+// out = exclusive scan of in.  The tile goes through LDS (padded against
+// bank conflicts) so global loads and stores are coalesced while thread t
+// scans the 16 consecutive items [t*16, t*16+16).
+__device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
+
 __global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
                                                              const uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kSortItems;
+    __shared__ uint32_t s_t[kScanTile + kScanTile / 32];
+    const uint32_t tile0 = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t idx = k * kSortThreads + threadIdx.x;
+        s_t[scan_pad(idx)] = (tile0 + idx < n) ? in[tile0 + idx] : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = threadIdx.x * kSortItems;
     uint32_t v[kSortItems];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t i = base + k;
-        v[k] = (i < n) ? in[i] : 0u;
+        v[k] = s_t[scan_pad(base + k)];
         sum += v[k];
     }
     uint32_t total;
     uint32_t run = sums[blockIdx.x] + block_excl_scan(sum, s_wave, &total);
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t i = base + k;
-        if (i < n) out[i] = run;
+        s_t[scan_pad(base + k)] = run;
         run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t idx = k * kSortThreads + threadIdx.x;
+        if (tile0 + idx < n) out[tile0 + idx] = s_t[scan_pad(idx)];
     }
 }
 

@@ -509,38 +509,34 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
         s_bb[wv][3] = b3;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // one atomic set for the waves that share the first wave's world, one per other wave
-        float acc[8];
-        for (int t = 0; t < 8; ++t) acc[t] = 0.0f;
-        uint32_t ab[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u};
+    // the block's waves: those sharing the first wave's world are combined into one
+    // update, each other uniform wave updates its own world; lane q of the first wave
+    // issues quantity q (12 quantities: 8 sums, 4 box bounds) so the atomics overlap
+    if (threadIdx.x < 12) {
+        const int q = threadIdx.x;
+        const uint32_t rsel = blockIdx.x & (rep - 1);
         const uint32_t wb = s_w[0];
+        float facc = 0.0f;
+        uint32_t uacc = q < 10 ? 0xFFFFFFFFu : 0u;
+        bool have = false;
         for (int k = 0; k < kThreads / BG_WAVE; ++k) {
-            if (s_w[k] == kNil) continue;
+            if (s_w[k] == kNil || !(s_part[k][7] > 0.0f)) continue;
             if (s_w[k] == wb) {
-                for (int t = 0; t < 8; ++t) acc[t] += s_part[k][t];
-                ab[0] = min(ab[0], s_bb[k][0]);
-                ab[1] = min(ab[1], s_bb[k][1]);
-                ab[2] = max(ab[2], s_bb[k][2]);
-                ab[3] = max(ab[3], s_bb[k][3]);
-            } else if (s_part[k][7] > 0.0f) {
-                float* d2 = wstat + ((size_t)s_w[k] * rep + (blockIdx.x & (rep - 1))) * 8;
-                for (int t = 0; t < 8; ++t) atomicAdd(&d2[t], s_part[k][t]);
-                uint32_t* bb = wbbox + ((size_t)s_w[k] * rep + (blockIdx.x & (rep - 1))) * 4;
-                atomic_min_if_less(&bb[0], s_bb[k][0]);
-                atomic_min_if_less(&bb[1], s_bb[k][1]);
-                atomic_max_if_greater(&bb[2], s_bb[k][2]);
-                atomic_max_if_greater(&bb[3], s_bb[k][3]);
+                have = true;
+                if (q < 8) facc += s_part[k][q];
+                else if (q < 10) uacc = min(uacc, s_bb[k][q - 8]);
+                else uacc = max(uacc, s_bb[k][q - 8]);
+            } else {
+                const uint32_t wk = s_w[k];
+                if (q < 8) atomicAdd(&wstat[((size_t)wk * rep + rsel) * 8 + q], s_part[k][q]);
+                else if (q < 10) atomic_min_if_less(&wbbox[((size_t)wk * rep + rsel) * 4 + (q - 8)], s_bb[k][q - 8]);
+                else atomic_max_if_greater(&wbbox[((size_t)wk * rep + rsel) * 4 + (q - 8)], s_bb[k][q - 8]);
             }
         }
-        if (wb != kNil && acc[7] > 0.0f) {
-            float* d2 = wstat + ((size_t)wb * rep + (blockIdx.x & (rep - 1))) * 8;
-            for (int t = 0; t < 8; ++t) atomicAdd(&d2[t], acc[t]);
-            uint32_t* bb = wbbox + ((size_t)wb * rep + (blockIdx.x & (rep - 1))) * 4;
-            atomic_min_if_less(&bb[0], ab[0]);
-            atomic_min_if_less(&bb[1], ab[1]);
-            atomic_max_if_greater(&bb[2], ab[2]);
-            atomic_max_if_greater(&bb[3], ab[3]);
+        if (have) {
+            if (q < 8) atomicAdd(&wstat[((size_t)wb * rep + rsel) * 8 + q], facc);
+            else if (q < 10) atomic_min_if_less(&wbbox[((size_t)wb * rep + rsel) * 4 + (q - 8)], uacc);
+            else atomic_max_if_greater(&wbbox[((size_t)wb * rep + rsel) * 4 + (q - 8)], uacc);
         }
     }
 }
