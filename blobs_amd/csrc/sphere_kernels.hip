@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kCoarseShift = 2;  // outlier table cells are 4x4 grid cells
 constexpr int kFlatRows = 4;      // grid rows a query walks as one flat candidate list (<= 64 candidates)
+constexpr int kContacts = 4;      // contacts summed from registers; more take a selection pass in key order
 
 struct SphereSubstep {
     SphereArrays src;  // body records read (through perm on the first substep after a rebuild)
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
                 }
                 put(idb, cx, cy);
             };
-            ContactList<8, uint32_t> L;
+            ContactList<kContacts, uint32_t> L;
             L.init();
             enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs) {
                 if (isA) ++count;
@@ -328,8 +329,8 @@ __global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, Su
             });
             if (L.n > 0) {
                 uint32_t last = L.apply(st.p);
-                uint32_t done = L.n < 8 ? L.n : 8;
-                while (done < L.n) {  // more than 8: next key by selection, same order
+                uint32_t done = L.n < (uint32_t)kContacts ? L.n : (uint32_t)kContacts;
+                while (done < L.n) {  // more than kContacts: next key by selection, same order
                     uint32_t best = kNil;
                     float bx = 0.0f, by = 0.0f;
                     enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs) {
