@@ -117,7 +117,7 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
 }
 
 template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER, bool BATCHED>
-__global__ __launch_bounds__(kThreads) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kThreads + threadIdx.x;
     // clear the outlier table of the substep after next
