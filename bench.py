@@ -169,6 +169,28 @@ def differentiable_leg(n_worlds: int, n_steps: int, ck_every: int) -> dict:
     return rec
 
 
+def multi_collider_leg(steps: int = 20, warmup: int = 3) -> dict:
+    """The multi-collider scene on the general path: 65,536 bodies with 1-8
+    sphere colliders each, 32,768 springs, 16,384 fixed joints, step(1/60)."""
+    from blobs_amd import scenes
+    sc = scenes.cfg3_multi()
+    p, _ = sc.build_physics()
+    p.step(sc.dt, warmup)
+    p._lib.bg_synchronize(p._w)
+    t0 = time.perf_counter()
+    p.step(sc.dt, steps)
+    p._lib.bg_synchronize(p._w)
+    el = time.perf_counter() - t0
+    rec = {
+        "workload": "65536 bodies x U{1..8} colliders, 32768 springs, 16384 fixed joints, step(1/60), general path",
+        "bodies": sc.n_bodies, "colliders": sc.n_colliders, "steps": steps,
+        "ms_per_step": 1000.0 * el / steps, "value": sc.n_bodies * steps / el, "unit": "body-steps/s",
+        "path": p.step_path(),
+    }
+    log(f"multi-collider: {rec['ms_per_step']:.2f} ms/step")
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +206,7 @@ def main() -> int:
                     help="also run a 256 MiB device copy (hardware byte-counter calibration)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-differentiable", action="store_true")
+    ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-chunks", type=int, default=85,
                     help="timed 10-step chunks (with warm-up and tuning the run stays within 1000 steps)")
@@ -304,6 +327,8 @@ def main() -> int:
         line["batched"] = b
     if not args.no_differentiable:
         line["differentiable"] = differentiable_leg(256, 200, 20)
+    if not args.no_multi:
+        line["multi_collider"] = multi_collider_leg()
     if not args.no_cpu:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)

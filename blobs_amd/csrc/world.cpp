@@ -1227,6 +1227,9 @@ int World::launch_steps(double delta, uint32_t n) {
             if (st) return st;
             fresh_ = false;
             stepped_since_upload_ = true;
+            // the general path widens every query by the largest displacement since its
+            // rebuild (one fast collider would slow every query): it re-sorts every substep
+            if (path_ == StepPath::General) need_rebuild_ = true;
         }
         if (++steps_since_rebuild_ >= rebuild_interval_) need_rebuild_ = true;
     }
