@@ -132,6 +132,19 @@ class ColliderBatchC(C.Structure):
     ]
 
 
+class CollisionEventC(C.Structure):
+    _fields_ = [
+        ("col_handle_a", handle_t),
+        ("col_handle_b", handle_t),
+        ("impact_vel_a", C.c_float * 2),
+        ("impact_vel_b", C.c_float * 2),
+    ]
+
+
+class Affine2C(C.Structure):
+    _fields_ = [("matrix", C.c_float * 4), ("translation", C.c_float * 2)]
+
+
 # (name, argtypes) of every entry point in include/blobs_gpu.h
 WORLD = C.c_void_p
 SIGNATURES = {
@@ -173,6 +186,9 @@ SIGNATURES = {
     "bg_batch_writeback": [WORLD, C.POINTER(WORLD), C.c_uint32],
     "bg_batch_size": [WORLD, u32p],
     "bg_autotune": [WORLD, C.c_double, C.c_uint32, f32p, f32p, u32p],
+    "bg_set_event_capacity": [WORLD, C.c_uint32],
+    "bg_drain_events": [WORLD, C.POINTER(CollisionEventC), C.c_uint32, u32p, u64p],
+    "bg_debug_data": [WORLD, C.POINTER(Affine2C), C.POINTER(Affine2C), f32p, f32p, f32p, u32p],
 }
 
 ROLLOUT = C.c_void_p

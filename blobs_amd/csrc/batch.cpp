@@ -185,6 +185,15 @@ int World::step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks) {
     };
     const bool prof = profiling_;
     uint32_t done = 0;
+    if (event_cap_ > 0) {
+        // events are stamped with the substep index the host passes: a replayed graph would
+        // repeat the stamps, so a world that records events runs its chunks eagerly
+        st = launch_steps(delta, chunk_steps * n_chunks);
+        if (st) return st;
+        advance_time(delta, chunk_steps * n_chunks);
+        host_stale_ = true;
+        return check_errors();
+    }
     // the first chunk after an upload runs eagerly (user inputs are consumed once)
     if (fresh_) {
         st = launch_steps(delta, chunk_steps);

@@ -265,6 +265,22 @@ int bg_last_error(bg_world* w, char* buf, size_t len) {
     return BG_OK;
 }
 
+int bg_set_event_capacity(bg_world* w, uint32_t capacity) {
+    return guarded(w, [&](bg::World& x) { return x.set_event_capacity(capacity); });
+}
+
+int bg_drain_events(bg_world* w, bg_collision_event* out, uint32_t cap, uint32_t* n_out, uint64_t* n_dropped) {
+    if (cap > 0 && !out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.drain_events(out, cap, n_out, n_dropped); });
+}
+
+int bg_debug_data(bg_world* w, bg_affine2* bodies, bg_affine2* colliders, float* collider_radius, float* joint_bodies,
+                  float* spring_bodies, uint32_t counts[4]) {
+    return guarded(w, [&](bg::World& x) {
+        return x.debug_data(bodies, colliders, collider_radius, joint_bodies, spring_bodies, counts);
+    });
+}
+
 int bg_set_broadphase(bg_world* w, float cell_scale, float skin, uint32_t rebuild_interval) {
     return guarded(w, [&](bg::World& x) { return x.set_broadphase(cell_scale, skin, rebuild_interval); });
 }

@@ -124,11 +124,19 @@ struct StepAccum {
     uint32_t pad0;
     uint32_t bbox[2][4];  // [parity]{minx, miny, maxx, maxy} as f2ord
     uint32_t n_outliers;  // sphere path: bodies beyond the skin since the last rebuild
-    uint32_t pad2;
+    uint32_t event_count;  // collision events recorded since the last drain (may exceed the buffer)
     uint32_t error_flags;
     uint32_t push_outs;
     float old_dt;  // Physics::old_dt (physics.rs:33), world scalar
     uint32_t pad1;
+};
+
+// One collision event (physics.rs:302-311): the substep it happened in, the
+// dense indices of collider A (the higher index) and B, and the bodies'
+// calculated_velocity before the contact pass.
+struct DevEvent {
+    uint32_t sub, a, b, pad;
+    float2 va, vb;
 };
 
 enum : uint32_t {

@@ -111,6 +111,11 @@ struct DeviceWorld {
     uint32_t n_block_slots = 0;
     SortTemp sort{};
 
+    // collision events (optional): buffer of event_cap records, count in accum
+    DevEvent *events = nullptr;
+    uint32_t event_cap = 0;
+    float2 *vel_next = nullptr;  // sphere path with events: new velocities (the old ones stay readable)
+
     hipStream_t stream = nullptr;
     int device = -1;
     bool allocated = false;
@@ -132,6 +137,7 @@ struct SubstepArgs {
     uint32_t parity;    // which bbox accumulator this substep writes
     uint32_t has_springs;
     uint32_t sub;       // sphere path: substeps since the last rebuild (1-based)
+    uint32_t sub_global;  // substeps since the world was created (orders collision events)
 };
 
 // Launchers (step_kernels.hip: general path + broadphase build).

@@ -68,6 +68,9 @@ struct SphereSubstep {
     const float2* __restrict__ con_pos;
     const float* __restrict__ con_r;
     uint32_t nk;
+    DevEvent* events;   // null: no event recording
+    uint32_t event_cap;
+    float2* vel_out;    // where this substep's velocities go
 };
 
 __device__ __forceinline__ int32_t free_cell(float x, float o, float inv) {
@@ -327,6 +330,26 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 correction(true, [&](uint32_t kk, float x, float y) { L.add(kk, x, y); }, t, idb, isA, ax, ay, dist,
                            rs);
             });
+            if (k.events && count > 0) {
+                // collision events of the pairs this body is A in, with both bodies'
+                // velocities before the contact pass (physics.rs:288-311)
+                uint32_t e = atomicAdd(&k.accum->event_count, count);
+                const float2 va = k.src.vel[g_own];
+                enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs) {
+                    if (!isA) return;
+                    if (e < k.event_cap) {
+                        DevEvent ev;
+                        ev.sub = a.sub_global;
+                        ev.a = id;
+                        ev.b = idb;
+                        ev.pad = 0;
+                        ev.va = va;
+                        ev.vb = k.src.vel[GATHER ? k.perm[t] : t];
+                        k.events[e] = ev;
+                    }
+                    ++e;
+                });
+            }
             if (L.n > 0) {
                 uint32_t last = L.apply(st.p);
                 uint32_t done = L.n < (uint32_t)kContacts ? L.n : (uint32_t)kContacts;
@@ -396,7 +419,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
             if (fl_out != fl) k.dst.bflags[s] = fl_out;
             if (a.fresh && !(fl & BF_STATIC)) k.dst.torque[s] = 0.0f;
         }
-        k.dst.vel[s] = vel;
+        k.vel_out[s] = vel;
         // collider transform from the pre-constraint position (physics.rs:360-366)
         const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform_slow(st.rot, st.p, k.src.off[g_own]);
         const bool finite_x = finitef(x.x) && finitef(x.y);
@@ -757,6 +780,10 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.con_pos = d.con_pos;
     k.con_r = d.con_r;
     k.nk = d.nk;
+    k.events = (d.events && a.collisions) ? d.events : nullptr;
+    k.event_cap = d.event_cap;
+    // with events the old velocities stay readable: the new ones go to vel_next
+    k.vel_out = k.events ? d.vel_next : k.dst.vel;
     const uint32_t blocks = div_up(d.nb, kThreads);
 #define BG_SPHERE(G, S, Z, C, P, B) k_sphere_substep<G, S, Z, C, P, B><<<blocks, kThreads, 0, d.stream>>>(k, a)
 #define BG_SPHERE_PB(G, S, Z, C)                 \
@@ -790,6 +817,11 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
         d.s.snap = snap_cur;
         d.s_alt.snap = nullptr;
         d.gather_pending = false;
+    }
+    if (k.events) {  // the current velocities are in vel_next now
+        float2* cur = k.vel_out;
+        d.vel_next = d.s.vel;
+        d.s.vel = cur;
     }
     float4* t = d.s.snap;
     d.s.snap = d.snap_next;

@@ -212,6 +212,27 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
             correction(true, [&](uint64_t k, float x, float y) { L.add(k, x, y); }, key, isA, j, pj, ax, ay, dist,
                        rs, sen_a);
         });
+        if (d.events && count > 0) {
+            // collision events of the pairs whose A collider is this body's (physics.rs:288-311);
+            // velocities are the bodies' calculated_velocity before the contact pass
+            uint32_t e = atomicAdd(&d.accum->event_count, count);
+            const float2 va = d.b.vel[b];
+            enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist, float rs,
+                          bool sen_a) {
+                if (!isA) return;
+                if (e < d.event_cap) {
+                    DevEvent ev;
+                    ev.sub = a.sub_global;
+                    ev.a = (uint32_t)(key >> 32);
+                    ev.b = (uint32_t)key;
+                    ev.pad = 0;
+                    ev.va = va;
+                    ev.vb = d.b.vel[pj];
+                    d.events[e] = ev;
+                }
+                ++e;
+            });
+        }
         if (L.n > 0) {
             uint64_t last = L.apply(p);
             uint32_t done = L.n < 16 ? L.n : 16;

@@ -491,6 +491,131 @@ int World::contact_count(uint64_t* out) {
     return BG_OK;
 }
 
+int World::set_event_capacity(uint32_t cap) {
+    if (sync_host()) return BG_ERR_HIP;
+    if (int st = collect_events()) return st;
+    event_cap_ = cap;
+    mark_dirty();  // buffers are sized at upload
+    return BG_OK;
+}
+
+// Device events -> host queue, in the reference's order.
+int World::collect_events() {
+    DeviceWorld& d = d_;
+    if (!d.allocated || !d.events) return BG_OK;
+    uint32_t n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, &d.accum->event_count, sizeof(uint32_t), hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    if (n == 0) return BG_OK;
+    const uint32_t kept = n < d.event_cap ? n : d.event_cap;
+    events_dropped_ += n - kept;
+    std::vector<DevEvent> ev;
+    HIP_TRY(d2h(ev, d.events, kept, d.stream));
+    HIP_TRY(hipMemsetAsync(&d.accum->event_count, 0, sizeof(uint32_t), d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    std::sort(ev.begin(), ev.end(), [](const DevEvent& x, const DevEvent& y) {
+        if (x.sub != y.sub) return x.sub < y.sub;
+        if (x.a != y.a) return x.a < y.a;
+        return x.b < y.b;
+    });
+    if (event_head_ > 0 && event_head_ == event_queue_.size()) {
+        event_queue_.clear();
+        event_head_ = 0;
+    }
+    for (const DevEvent& e : ev) {
+        bg_collision_event o;
+        o.col_handle_a = colliders_.handle_at(col_slot_of_dense_[e.a]);
+        o.col_handle_b = colliders_.handle_at(col_slot_of_dense_[e.b]);
+        o.impact_vel_a[0] = e.va.x;
+        o.impact_vel_a[1] = e.va.y;
+        o.impact_vel_b[0] = e.vb.x;
+        o.impact_vel_b[1] = e.vb.y;
+        event_queue_.push_back(o);
+    }
+    return BG_OK;
+}
+
+int World::drain_events(bg_collision_event* out, uint32_t cap, uint32_t* n_out, uint64_t* n_dropped) {
+    if (int st = collect_events()) return st;
+    const size_t avail = event_queue_.size() - event_head_;
+    const size_t n = avail < cap ? avail : cap;
+    if (n && out) memcpy(out, event_queue_.data() + event_head_, n * sizeof(bg_collision_event));
+    event_head_ += n;
+    if (event_head_ == event_queue_.size()) {
+        event_queue_.clear();
+        event_head_ = 0;
+    }
+    if (n_out) *n_out = (uint32_t)n;
+    if (n_dropped) *n_dropped = events_dropped_;
+    events_dropped_ = 0;
+    return BG_OK;
+}
+
+// Physics::debug_data, debug.rs:34-91 (from the host mirror).
+int World::debug_data(bg_affine2* bodies, bg_affine2* cols, float* radius, float* joints, float* springs,
+                      uint32_t* counts) {
+    if (sync_host()) return BG_ERR_HIP;
+    uint32_t nb = 0, nc = 0, nj = 0, ns = 0;
+    for (uint32_t s = 0; s < bodies_.capacity_slots(); ++s) {
+        if (!bodies_.occupied(s)) continue;
+        const bg_rigid_body& f = bodies_.at_slot(s).f;
+        if (bodies) {
+            // Affine2::from_angle_translation: Mat2::from_angle = cols (cos, sin), (-sin, cos)
+            const float sn = sinf(f.rotation), cs = cosf(f.rotation);
+            bg_affine2& t = bodies[nb];
+            t.matrix[0] = cs;
+            t.matrix[1] = sn;
+            t.matrix[2] = -sn;
+            t.matrix[3] = cs;
+            t.translation[0] = f.position[0];
+            t.translation[1] = f.position[1];
+        }
+        ++nb;
+    }
+    for (uint32_t s = 0; s < colliders_.capacity_slots(); ++s) {
+        if (!colliders_.occupied(s)) continue;
+        const bg_collider& c = colliders_.at_slot(s).f;
+        if (cols) {
+            memcpy(cols[nc].matrix, c.absolute_matrix, sizeof(cols[nc].matrix));
+            memcpy(cols[nc].translation, c.absolute_translation, sizeof(cols[nc].translation));
+        }
+        if (radius) radius[nc] = c.radius;
+        ++nc;
+    }
+    auto ends = [&](bg_handle a, bg_handle b, float* out4) -> bool {
+        const HostBody* ba = bodies_.get(a);
+        const HostBody* bb = bodies_.get(b);
+        if (!ba || !bb) return false;  // the reference indexes the arena and panics
+        out4[0] = ba->f.position[0];
+        out4[1] = ba->f.position[1];
+        out4[2] = bb->f.position[0];
+        out4[3] = bb->f.position[1];
+        return true;
+    };
+    float tmp[4];
+    for (uint32_t s = 0; s < joints_.capacity_slots(); ++s) {
+        if (!joints_.occupied(s)) continue;
+        const bg_fixed_joint& j = joints_.at_slot(s);
+        if (!ends(j.rigid_body_a, j.rigid_body_b, joints ? joints + 4 * nj : tmp))
+            return fail(BG_ERR_PANIC, "joint body missing (debug.rs:48-49 index)");
+        ++nj;
+    }
+    for (uint32_t s = 0; s < springs_.capacity_slots(); ++s) {
+        if (!springs_.occupied(s)) continue;
+        const bg_spring& sp = springs_.at_slot(s);
+        if (!ends(sp.rigid_body_a, sp.rigid_body_b, springs ? springs + 4 * ns : tmp))
+            return fail(BG_ERR_PANIC, "spring body missing (debug.rs:79-80 index)");
+        ++ns;
+    }
+    if (counts) {
+        counts[0] = nb;
+        counts[1] = nc;
+        counts[2] = nj;
+        counts[3] = ns;
+    }
+    return BG_OK;
+}
+
 int World::set_broadphase(float cell_scale, float skin, uint32_t rebuild_interval) {
     if (!(cell_scale >= 0.25f) || !(skin > 0.0f && skin <= 4.0f) || rebuild_interval == 0)
         return fail(BG_ERR_INVALID_ARGUMENT,
@@ -539,7 +664,7 @@ void World::free_arrays() {
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
     dfree(d.jc_start), dfree(d.jc_list), dfree(d.con_pos), dfree(d.con_r);
     dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start);
-    dfree(d.grid), dfree(d.accum), dfree(d.block_contacts);
+    dfree(d.grid), dfree(d.accum), dfree(d.block_contacts), dfree(d.events), dfree(d.vel_next);
     sort_temp_free(d.sort);
     hipStream_t s = d.stream;
     int dev = d.device;
@@ -558,12 +683,14 @@ void World::free_device() {
 // reference's iteration order) and copy them to the device.
 int World::upload() {
     if (d_.stream) HIP_TRY(hipStreamSynchronize(d_.stream));
-    // keep the contact counter across re-uploads
+    // keep the contact counter and pending collision events across re-uploads
     {
         uint64_t dev = 0;
         int st = device_contacts(&dev);
         if (st) return st;
         contact_base_ += dev;
+        st = collect_events();
+        if (st) return st;
     }
     free_arrays();
 
@@ -977,6 +1104,11 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.cell_start, (size_t)cell_rows + 1));
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
+    if (event_cap_ > 0) {
+        d.event_cap = event_cap_;
+        HIP_TRY(dmalloc(&d.events, event_cap_));
+        if (sphere) HIP_TRY(dmalloc(&d.vel_next, nb));
+    }
     d.n_block_slots = (std::max<uint32_t>(nb, 1) + 255) / 256;
     HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
     HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
@@ -1170,6 +1302,7 @@ int World::run_substep(float h, bool fresh) {
     a.has_springs = d.ns > 0 ? 1u : 0u;
     a.skin = skin_;
     a.sub = ++sub_since_rebuild_;
+    a.sub_global = substep_seq_++;
     a.t_prev = (float)((double)(a.sub - 1) * (double)h);
     a.t_cur = (float)((double)a.sub * (double)h);
     if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;

@@ -97,6 +97,10 @@ public:
     int get_collider_positions(float* abs, uint32_t* count);
     int contact_count(uint64_t* out);
     int set_broadphase(float cell_scale, float skin, uint32_t rebuild_interval);
+    int set_event_capacity(uint32_t cap);
+    int drain_events(bg_collision_event* out, uint32_t cap, uint32_t* n_out, uint64_t* n_dropped);
+    int debug_data(bg_affine2* bodies, bg_affine2* cols, float* radius, float* joints, float* springs,
+                   uint32_t* counts);
     int set_profiling(bool enable);
     int debug_accum(uint32_t* out4);
     int stage_stats(uint32_t stage, uint64_t* launches, double* total_ms);
@@ -130,6 +134,7 @@ private:
     int stage_begin(uint32_t stage);
     int stage_end(uint32_t stage);
     int collect_stage_times();
+    int collect_events();
 
     Arena<HostBody> bodies_;
     Arena<HostCollider> colliders_;
@@ -165,6 +170,12 @@ private:
     // broadphase drift of the current rebuild period
     double drift_x_ = 0, drift_y_ = 0, vdrift_x_ = 0, vdrift_y_ = 0, h_prev_ = 0;
     uint64_t contact_base_ = 0;
+    // collision events: device capacity, host queue (reference order), losses
+    uint32_t event_cap_ = 0;
+    uint32_t substep_seq_ = 0;
+    std::vector<bg_collision_event> event_queue_;
+    size_t event_head_ = 0;
+    uint64_t events_dropped_ = 0;
     // batch: dense body range, old_dt and time of each source world
     std::vector<uint32_t> batch_start_;
     std::vector<float> batch_old_dt_;

@@ -549,6 +549,58 @@ class Physics:
         (fraction of a cell) beyond which a body is tracked as an outlier."""
         self._chk(self._lib.bg_set_broadphase(self._w, float(cell_scale), float(skin), int(rebuild_interval)))
 
+    def set_event_capacity(self, capacity: int) -> None:
+        """Record collision events (physics.rs:304-311) into a device buffer of
+        `capacity` entries between drains; 0 turns recording off."""
+        self._chk(self._lib.bg_set_event_capacity(self._w, int(capacity)))
+
+    def drain_events(self, max_events: int | None = None):
+        """Drain `collision_recv`: (handles [n, 2] uint64 as (col_handle_a,
+        col_handle_b), impact velocities [n, 4] float32 as (vel_a, vel_b),
+        events dropped to a full buffer since the last drain), in the order the
+        reference sends them."""
+        chunk = 1 << 16
+        hs, vs = [], []
+        dropped = 0
+        left = max_events
+        while left is None or left > 0:
+            cap = chunk if left is None else min(chunk, left)
+            buf = (L.CollisionEventC * cap)()
+            n = C.c_uint32()
+            d = C.c_uint64()
+            self._chk(self._lib.bg_drain_events(self._w, buf, cap, C.byref(n), C.byref(d)))
+            dropped += d.value
+            arr = np.frombuffer(buf, dtype=np.dtype([("a", np.uint64), ("b", np.uint64), ("va", np.float32, 2),
+                                                     ("vb", np.float32, 2)]), count=n.value)
+            hs.append(np.stack([arr["a"], arr["b"]], 1))
+            vs.append(np.concatenate([arr["va"], arr["vb"]], 1))
+            if left is not None:
+                left -= n.value
+            if n.value < cap:
+                break
+        return np.concatenate(hs).reshape(-1, 2), np.concatenate(vs).reshape(-1, 4), dropped
+
+    def debug_data(self) -> dict:
+        """Physics::debug_data (debug.rs:34-91) as arrays in arena order:
+        body and collider transforms as (column-major Mat2, translation),
+        collider radii, and (body_a, body_b) positions of joints and springs."""
+        cnt = np.zeros(4, np.uint32)
+        self._chk(self._lib.bg_debug_data(self._w, None, None, None, None, None, L.u32ptr(cnt)))
+        nb, nc, nj, ns = (int(x) for x in cnt)
+        bodies = (L.Affine2C * max(nb, 1))()
+        cols = (L.Affine2C * max(nc, 1))()
+        rad = np.zeros(nc, np.float32)
+        joints = np.zeros((nj, 4), np.float32)
+        springs = np.zeros((ns, 4), np.float32)
+        self._chk(self._lib.bg_debug_data(self._w, bodies, cols, L.fptr(rad), L.fptr(joints), L.fptr(springs),
+                                          L.u32ptr(cnt)))
+        aff = np.dtype([("m", np.float32, 4), ("t", np.float32, 2)])
+        b = np.frombuffer(bodies, dtype=aff, count=nb)
+        c = np.frombuffer(cols, dtype=aff, count=nc)
+        return {"body_matrix": b["m"].copy(), "body_translation": b["t"].copy(), "collider_matrix": c["m"].copy(),
+                "collider_translation": c["t"].copy(), "collider_radius": rad, "joints": joints,
+                "springs": springs}
+
     def set_profiling(self, enable: bool = True) -> None:
         self._chk(self._lib.bg_set_profiling(self._w, int(bool(enable))))
 

@@ -241,6 +241,40 @@ int bg_contact_count(bg_world *w, uint64_t *out);
 /* Text of the last error on this world (or the global one if w is NULL). */
 int bg_last_error(bg_world *w, char *buf, size_t len);
 
+/* ---- collision events and debug data ---------------------------------- */
+
+/* One CollisionEvent (lib.rs:146-153): collider A is the later one in
+ * iteration order; velocities are the parents' calculated_velocity before
+ * the contact pass (physics.rs:288-289). */
+typedef struct {
+    bg_handle col_handle_a, col_handle_b;
+    float impact_vel_a[2], impact_vel_b[2];
+} bg_collision_event;
+
+/* The reference sends every contact into the world's mpsc channel
+ * (physics.rs:38, 304-311).  Recording is off by default; with capacity
+ * > 0 the device keeps up to `capacity` events between drains (more are
+ * counted as dropped).  Re-uploads keep pending events. */
+int bg_set_event_capacity(bg_world *w, uint32_t capacity);
+/* Physics::collision_recv drained: up to `cap` events in the reference's
+ * order (substep, then A, then B -- the (i, j<i) loop of physics.rs:248-249);
+ * the rest stay queued.  *n_dropped: events lost to a full buffer since the
+ * last drain. */
+int bg_drain_events(bg_world *w, bg_collision_event *out, uint32_t cap, uint32_t *n_out, uint64_t *n_dropped);
+
+typedef struct {
+    float matrix[4];     /* column-major Mat2 */
+    float translation[2];
+} bg_affine2;
+
+/* Physics::debug_data (physics.rs:479-481, debug.rs:34-91): bodies
+ * (Affine2::from_angle_translation(rotation, position)), colliders
+ * (absolute_transform, radius), joints and springs (positions of their two
+ * bodies, 4 floats each), all in arena order.  Any output may be NULL;
+ * counts receives {bodies, colliders, joints, springs}. */
+int bg_debug_data(bg_world *w, bg_affine2 *bodies, bg_affine2 *colliders, float *collider_radius,
+                  float *joint_bodies, float *spring_bodies, uint32_t counts[4]);
+
 /* ---- tuning ---------------------------------------------------------- */
 
 /* Broadphase tuning: grid cell size as a multiple of 2*max collider radius

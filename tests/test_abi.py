@@ -37,7 +37,8 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     structs = {"bg_rigid_body": L.RigidBodyC, "bg_collider": L.ColliderC, "bg_spring": L.SpringC,
                "bg_fixed_joint": L.FixedJointC, "bg_params": L.ParamsC, "bg_body_batch": L.BodyBatchC,
-               "bg_collider_batch": L.ColliderBatchC}
+               "bg_collider_batch": L.ColliderBatchC, "bg_collision_event": L.CollisionEventC,
+               "bg_affine2": L.Affine2C}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "blobs_gpu.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -217,3 +218,55 @@ def test_batch_run_without_device_fails_loudly():
     with pytest.raises(L.BlobsError) as e:
         b.run_chunks(1 / 60, 10, 2)
     assert e.value.code == L.BG_ERR_NO_DEVICE
+
+
+def test_debug_data_on_host():
+    """Physics::debug_data (debug.rs:34-91) before any step: body transforms
+    Affine2::from_angle_translation(rotation, position) (Mat2::from_angle is
+    columns (cos, sin), (-sin, cos)), collider absolute transforms and radii,
+    joint and spring body positions, each in arena order."""
+    p = Physics((0.0, -9.81), False)
+    rot = [0.0, np.pi / 2, 0.3, -1.2]
+    hs = [p.insert_rbd(RigidBodyBuilder().position((float(k), 2.0 * k)).rotation(rot[k]).build()) for k in range(4)]
+    cs = [p.insert_collider_with_parent(ColliderBuilder().radius(0.1 * (k + 1)).offset_translation((0.5, 0)).build(),
+                                        hs[k]) for k in range(4)]
+    p.create_fixed_joint(hs[0], hs[1], (0, 0), (0, 0))
+    p.springs.insert(Spring(hs[3], hs[2], 1.0, 10.0, 0.0))
+    p.remove_rbd(hs[1])  # removes its collider and leaves the joint dangling
+    with pytest.raises(L.BlobsError) as e:
+        p.debug_data()  # the reference indexes the arena and panics
+    assert e.value.code == L.BG_ERR_PANIC
+    p = Physics((0.0, -9.81), False)
+    hs = [p.insert_rbd(RigidBodyBuilder().position((float(k), 2.0 * k)).rotation(rot[k]).build()) for k in range(4)]
+    cs = [p.insert_collider_with_parent(ColliderBuilder().radius(0.1 * (k + 1)).offset_translation((0.5, 0)).build(),
+                                        hs[k]) for k in range(4)]
+    p.create_fixed_joint(hs[0], hs[1], (0, 0), (0, 0))
+    p.springs.insert(Spring(hs[3], hs[2], 1.0, 10.0, 0.0))
+    dd = p.debug_data()
+    r = np.array(rot, np.float32)
+    m = dd["body_matrix"]
+    assert np.allclose(m[:, 0], np.cos(r), atol=1e-7) and np.allclose(m[:, 1], np.sin(r), atol=1e-7)
+    assert np.array_equal(m[:, 2], -m[:, 1]) and np.array_equal(m[:, 3], m[:, 0])
+    assert np.array_equal(m[0], [1, 0, -0.0, 1])  # rotation 0 is the identity
+    assert np.allclose(m[1], [0, 1, -1, 0], atol=1e-7)
+    pos = np.array([[k, 2.0 * k] for k in range(4)], np.float32)
+    assert np.array_equal(dd["body_translation"], pos)
+    for k in range(4):
+        c = p.get_col(cs[k])
+        assert np.array_equal(dd["collider_translation"][k], np.float32(c.absolute_translation))
+        assert dd["collider_radius"][k] == np.float32(0.1 * (k + 1))
+    # a collider's absolute transform is the identity until the first step (collider.rs:215)
+    assert np.array_equal(dd["collider_matrix"], np.tile(np.float32([1, 0, 0, 1]), (4, 1)))
+    assert np.array_equal(dd["joints"], [np.concatenate([pos[0], pos[1]])])
+    assert np.array_equal(dd["springs"], [np.concatenate([pos[3], pos[2]])])
+
+
+def test_event_stream_without_device():
+    """Recording can be switched on before the first step; draining a world
+    that never ran yields nothing."""
+    p = Physics((0.0, -9.81), False)
+    h = p.insert_rbd(RigidBodyBuilder().position((0, 0)).build())
+    p.insert_collider_with_parent(ColliderBuilder().radius(0.5).build(), h)
+    p.set_event_capacity(1024)
+    hs, vs, dropped = p.drain_events()
+    assert hs.shape == (0, 2) and vs.shape == (0, 4) and dropped == 0

@@ -1,0 +1,174 @@
+"""Collision events and debug data on the GPU, against the oracle.
+
+The reference sends one CollisionEvent per contact into the world's channel
+(physics.rs:304-311): collider A is the later collider in arena order, and
+the impact velocities are both parents' calculated_velocity before the
+contact pass.  The GPU records events into a bounded device buffer; a drain
+sorts them into the reference's order (substep, then A, then B).  Bar:
+handles and order identical to the oracle's event list, velocities
+bit-identical (f32).
+"""
+import numpy as np
+import pytest
+
+from blobs_amd import scenes
+from oracle.oracle import OracleWorld
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def oracle_events(sc, steps, cap=1 << 21):
+    o = OracleWorld(sc, np.float32, event_cap=cap)
+    o.step(sc.dt, steps)
+    assert o.w.n_events <= cap
+    cols, vels = o.events
+    return o, cols.astype(np.int64), vels
+
+
+def gpu_events(sc, steps, cap=1 << 21, broadphase=None):
+    p, _ = sc.build_physics()
+    if broadphase:
+        p.set_broadphase(*broadphase)
+    p.set_event_capacity(cap)
+    p.step(sc.dt, steps)
+    hs, vs, dropped = p.drain_events()
+    return p, hs, vs, dropped
+
+
+def slots(hs):
+    # a fresh world's colliders sit in slots 0..n-1 in insertion order
+    return (hs & np.uint64(0xFFFFFFFF)).astype(np.int64)
+
+
+def assert_same_events(hs, vs, cols, vels):
+    assert hs.shape[0] == cols.shape[0], (hs.shape[0], cols.shape[0])
+    got = slots(hs)
+    bad = np.argwhere((got != cols).any(1))
+    assert bad.shape[0] == 0, f"{bad.shape[0]} events differ, first at {bad[0, 0]}: {got[bad[0, 0]]} vs {cols[bad[0, 0]]}"
+    bad = np.argwhere((bits(vs) != bits(vels)).any(1))
+    assert bad.shape[0] == 0, f"{bad.shape[0]} impact velocities differ, first at {bad[0, 0]}"
+
+
+@pytest.mark.parametrize("broadphase", [None, (1.25, 3, 0.25)])
+def test_sphere_path_events_match_oracle(broadphase):
+    sc = scenes.cfg1_plain_small(n=1024, steps=30, seed=11)
+    _, cols, vels = oracle_events(sc, 30)
+    p, hs, vs, dropped = gpu_events(sc, 30, broadphase=broadphase)
+    assert p.step_path() == "sphere"
+    assert dropped == 0
+    assert cols.shape[0] > 1000, "the scene should collide"
+    assert_same_events(hs, vs, cols, vels)
+    assert p.contact_count() == cols.shape[0]
+
+
+def test_general_path_events_match_oracle():
+    # multi-collider bodies and springs; no joints, whose rotations differ from
+    # the oracle's in the last bit of sinf/cosf/atan2f and would fork the trajectory
+    sc = scenes.cfg3_multi(n=1024, n_springs=512, n_joints=0, steps=10, seed=12, phi=0.3)
+    _, cols, vels = oracle_events(sc, 10)
+    p, hs, vs, dropped = gpu_events(sc, 10)
+    assert p.step_path() == "general"
+    assert dropped == 0
+    assert cols.shape[0] > 100
+    assert_same_events(hs, vs, cols, vels)
+
+
+def test_events_overflow_are_counted_and_kept_events_are_the_reference_s():
+    sc = scenes.cfg2_plain_large(n=4096, steps=5, seed=13)
+    _, cols, vels = oracle_events(sc, 5)
+    cap = 400
+    assert cols.shape[0] > 2 * cap
+    p, hs, vs, dropped = gpu_events(sc, 5, cap=cap)
+    assert hs.shape[0] == cap and dropped == cols.shape[0] - cap
+    # a pair collides again in later substeps with other velocities: key on both
+    pos = {}
+    for k, (ab, v) in enumerate(zip(cols, bits(vels))):
+        pos.setdefault((int(ab[0]), int(ab[1])) + tuple(int(x) for x in v), k)
+    got = slots(hs)
+    order = [pos.get((int(ab[0]), int(ab[1])) + tuple(int(x) for x in v), -1) for ab, v in zip(got, bits(vs))]
+    assert min(order) >= 0, "every kept event is one of the reference's"
+    assert order == sorted(order), "the drain is in reference order even when partial"
+    # after a drain the buffer is empty again
+    p.step(sc.dt, 1)
+    hs2, _, dropped2 = p.drain_events()
+    assert hs2.shape[0] <= cap
+
+
+def test_partial_drains_and_reupload_keep_the_stream():
+    sc = scenes.cfg2_plain_large(n=4096, steps=6, seed=14)
+    _, cols, vels = oracle_events(sc, 6)
+    p, _ = sc.build_physics()
+    p.set_event_capacity(1 << 20)
+    p.step(sc.dt, 2)
+    h1, v1, _ = p.drain_events(max_events=100)
+    assert h1.shape[0] == 100
+    p.set_event_capacity(1 << 20)  # re-upload; pending events stay queued
+    p.step(sc.dt, 4)
+    h2, v2, d = p.drain_events()
+    assert d == 0
+    assert_same_events(np.concatenate([h1, h2]), np.concatenate([v1, v2]), cols, vels)
+
+
+def test_events_off_by_default_and_after_zero_capacity():
+    sc = scenes.cfg2_plain_large(n=1024, steps=5, seed=15)
+    p, _ = sc.build_physics()
+    p.step(sc.dt, 5)
+    hs, _, d = p.drain_events()
+    assert hs.shape[0] == 0 and d == 0
+    p.set_event_capacity(1 << 16)
+    p.step(sc.dt, 2)
+    assert p.drain_events()[0].shape[0] > 0
+    p.set_event_capacity(0)
+    p.step(sc.dt, 2)
+    assert p.drain_events()[0].shape[0] == 0
+
+
+def test_batched_chunks_with_events_match_plain_steps():
+    ws = scenes.cfg4_scenes(n_worlds=6, lo=64, hi=512, seed=16)
+    steps = 6
+    # each world alone, events drained per world
+    alone = []
+    for sc in ws:
+        p, _ = sc.build_physics()
+        p.set_event_capacity(1 << 18)
+        p.step(sc.dt, steps)
+        alone.append(p.drain_events()[0].shape[0])
+    from blobs_amd.physics import Physics
+    b = Physics.batch([sc.build_physics()[0] for sc in ws])
+    b.set_event_capacity(1 << 20)
+    b.run_chunks(ws[0].dt, 3, 2)
+    hs, vs, d = b.drain_events()
+    assert d == 0
+    assert hs.shape[0] == sum(alone)
+
+
+def test_debug_data_after_steps():
+    """Physics::debug_data (debug.rs:34-91) from the stepped state."""
+    sc = scenes.cfg3_multi(n=256, n_springs=64, n_joints=32, steps=5, seed=17, phi=0.2)
+    p, bh = sc.build_physics()
+    p.step(sc.dt, 5)
+    dd = p.debug_data()
+    b = p.bodies()
+    assert np.array_equal(dd["body_translation"], b["position"])
+    r = b["rotation"]
+    m = dd["body_matrix"]
+    assert np.allclose(m[:, 0], np.cos(r), atol=2e-7) and np.allclose(m[:, 1], np.sin(r), atol=2e-7)
+    assert np.array_equal(m[:, 2], -m[:, 1]) and np.array_equal(m[:, 3], m[:, 0])
+    assert np.array_equal(dd["collider_translation"], p.collider_positions())
+    assert np.array_equal(dd["collider_radius"], sc.col_r)
+    pos = b["position"]
+    assert np.array_equal(dd["springs"], np.concatenate([pos[sc.spring_a], pos[sc.spring_b]], 1))
+    assert np.array_equal(dd["joints"], np.concatenate([pos[sc.joint_a], pos[sc.joint_b]], 1))
+    # collider matrices: body.transform() * offset at the last collider update
+    # (physics.rs:360-366); bodies without joints never rotate
+    cm = dd["collider_matrix"]
+    jointed = np.zeros(sc.n_bodies, bool)
+    jointed[sc.joint_a] = True
+    jointed[sc.joint_b] = True
+    free = ~jointed[sc.col_parent]
+    assert np.array_equal(cm[free], np.tile(np.float32([1, 0, 0, 1]), (int(free.sum()), 1)))
+    assert np.allclose(cm[:, 0] * cm[:, 3] - cm[:, 1] * cm[:, 2], 1.0, atol=1e-6)
