@@ -97,6 +97,47 @@ class RigidBody:
             self.acceleration = (float(np.float32(self.acceleration[0]) + np.float32(force[0]) / m),
                                  float(np.float32(self.acceleration[1]) + np.float32(force[1]) / m))
 
+    def _lever(self, world_point):
+        # world_point - (position + center_of_mass), f32 per component
+        f = np.float32
+        return (f(world_point[0]) - (f(self.position[0]) + f(self.center_of_mass[0])),
+                f(world_point[1]) - (f(self.position[1]) + f(self.center_of_mass[1])))
+
+    @staticmethod
+    def _perp_dot(a, b):
+        # glam Vec2::perp_dot: a.x * b.y - a.y * b.x
+        f = np.float32
+        return f(a[0]) * f(b[1]) - f(a[1]) * f(b[0])
+
+    def apply_impulse(self, impulse) -> None:
+        """rigid_body.rs:130-135: velocity_request = calculated_velocity + impulse / mass."""
+        if not self.is_static():
+            m = np.float32(self.calculated_mass)
+            self.add_velocity((np.float32(impulse[0]) / m, np.float32(impulse[1]) / m))
+
+    def apply_impulse_at_point(self, impulse, world_point) -> None:
+        """rigid_body.rs:137-148."""
+        if not self.is_static():
+            self.apply_impulse(impulse)
+            ang = self._perp_dot(self._lever(world_point), impulse)
+            self.angular_velocity = float(np.float32(self.angular_velocity) + ang / np.float32(self.inertia))
+
+    def add_velocity(self, velocity) -> None:
+        """rigid_body.rs:150-152."""
+        v = self.get_velocity()
+        self.set_velocity((np.float32(v[0]) + np.float32(velocity[0]), np.float32(v[1]) + np.float32(velocity[1])))
+
+    def apply_force_at_point(self, force, world_point) -> None:
+        """rigid_body.rs:162-171."""
+        if not self.is_static():
+            self.apply_force(force)
+            self.torque = float(np.float32(self.torque) + self._perp_dot(self._lever(world_point), force))
+
+    def apply_torque_at_point(self, force, world_point) -> None:
+        """rigid_body.rs:173-179: torque only."""
+        if not self.is_static():
+            self.torque = float(np.float32(self.torque) + self._perp_dot(self._lever(world_point), force))
+
     def set_velocity(self, v) -> None:
         """rigid_body.rs:182-184."""
         self.velocity_request = (float(v[0]), float(v[1]))

@@ -270,3 +270,27 @@ def test_event_stream_without_device():
     p.set_event_capacity(1024)
     hs, vs, dropped = p.drain_events()
     assert hs.shape == (0, 2) and vs.shape == (0, 4) and dropped == 0
+
+
+def test_rigid_body_force_and_impulse_helpers():
+    """rigid_body.rs:130-184 on the host value: closed forms with exact f32
+    values, perp_dot's sign (x * y' - y * x'), and static bodies untouched."""
+    from blobs_amd.physics import RigidBody
+    b = RigidBody(position=(1.0, 2.0), calculated_mass=2.0, inertia=4.0, calculated_velocity=(0.5, -1.0))
+    b.apply_force((4.0, -6.0))
+    assert b.acceleration == (2.0, -3.0)
+    b.apply_force_at_point((0.0, 2.0), (2.0, 2.0))  # lever (1, 0): torque +2
+    assert b.acceleration == (2.0, -2.0) and b.torque == 2.0
+    b.apply_torque_at_point((2.0, 0.0), (1.0, 3.0))  # lever (0, 1): torque -2
+    assert b.torque == 0.0 and b.acceleration == (2.0, -2.0)
+    b.apply_impulse((1.0, 2.0))  # velocity request = calculated velocity + J / m
+    assert b.velocity_request == (1.0, 0.0)
+    b.apply_impulse_at_point((0.0, 4.0), (3.0, 2.0))  # lever (2, 0): angular impulse 8, / inertia 4
+    assert b.velocity_request == (0.5, 1.0) and b.angular_velocity == 2.0
+    b.add_velocity((1.0, 1.0))
+    assert b.velocity_request == (1.5, 0.0)
+    s = RigidBody(body_type=STATIC)
+    s.apply_force((1.0, 1.0))
+    s.apply_impulse_at_point((1.0, 1.0), (5.0, 5.0))
+    s.apply_force_at_point((1.0, 1.0), (5.0, 5.0))
+    assert s.acceleration == (0.0, 0.0) and s.velocity_request is None and s.torque == 0.0
