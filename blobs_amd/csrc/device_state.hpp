@@ -50,6 +50,8 @@ struct SphereArrays {
 
 constexpr uint32_t kSnapOutlier = 0x80000000u;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
+constexpr uint32_t kNbrCap = 16;      // partner-list entries per body (sphere path)
+constexpr uint8_t kNbrNone = 0xFF;    // no list: the body walks the grid
 
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
@@ -64,6 +66,12 @@ struct DeviceWorld {
     // sphere path
     SphereArrays s{}, s_alt{};
     float4 *snap_next = nullptr;
+    // partner lists, written by the first substep after a rebuild: 16-bit
+    // slot deltas of the bodies whose rebuild snapshots lie within r_a + r_b
+    // + 2 skin, two a word, [kNbrCap/2][nb]; nbr_len[s] == kNbrNone: walk
+    // the grid instead
+    uint32_t *nbr = nullptr;
+    uint8_t *nbr_len = nullptr;
     // outlier cells: hashed linked lists (plus an occupancy bitmap) of the
     // bodies that moved more than the skin since the last rebuild, refilled
     // every substep (3 rotating tables)

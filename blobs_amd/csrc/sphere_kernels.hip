@@ -37,8 +37,10 @@ namespace bg {
 namespace {
 
 constexpr int kCoarseShift = 2;  // outlier table cells are 4x4 grid cells
-constexpr int kFlatRows = 4;      // grid rows a query walks as one flat candidate list (<= 64 candidates)
+constexpr int kFlatRows = 6;      // grid rows a query walks as one flat candidate list (<= 64 candidates)
 constexpr int kContacts = 4;      // contacts summed from registers; more take a selection pass in key order
+constexpr int kHitBatch = 2;      // marked candidates whose snapshot and mass load together
+constexpr int kSubstepWaves = 6;  // waves per SIMD the substep kernel is compiled for
 
 struct SphereSubstep {
     SphereArrays src;  // body records read (through perm on the first substep after a rebuild)
@@ -71,6 +73,9 @@ struct SphereSubstep {
     DevEvent* events;   // null: no event recording
     uint32_t event_cap;
     float2* vel_out;    // where this substep's velocities go
+    uint32_t* nbr;      // partner lists (two 16-bit slot deltas a word): written by the first
+                        // substep after a rebuild, read by the rest
+    uint8_t* nbr_len;
 };
 
 __device__ __forceinline__ int32_t free_cell(float x, float o, float inv) {
@@ -120,7 +125,7 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
 }
 
 template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER, bool BATCHED>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubstepWaves))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kThreads + threadIdx.x;
     // clear the outlier table of the substep after next
@@ -146,6 +151,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
         // loaded now, used after the contact pass: their latency overlaps the candidate walk
         const float2 rw = k.src.rw[g_own];
         const float2 x0 = k.x0[s];
+        const uint32_t nlen = GATHER ? (uint32_t)kNbrNone : (uint32_t)k.nbr_len[s];
+        // the first eight partner-list entries, loaded with the record (the rest on demand)
+        uint32_t lw[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) lw[j] = GATHER ? 0u : k.nbr[j * k.nb + s];
         const float mown = mgv.x;
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
@@ -160,8 +170,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
             // same expression as the skin test that wrote it)
             const float2 vw = k.wvel[w];
             const float dpx = a.drift_prev_x + vw.x * a.t_prev, dpy = a.drift_prev_y + vw.y * a.t_prev;
-            // reference predicate (physics.rs:255-269) against the slot-t snapshot
-            auto test = [&](uint32_t t, const float4 sb, auto&& sink) {
+            // reference predicate (physics.rs:255-269) against the slot-t snapshot; the
+            // partner's mass is `mp` when the caller loaded it (have_m), else loaded on a hit
+            auto test = [&](uint32_t t, const float4 sb, float mp, bool have_m, auto&& sink) {
                 if (GROUPS) {
                     const uint2 gb = k.src.groups[GATHER ? k.perm[t] : t];
                     if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) return;
@@ -176,18 +187,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 if (!(dist < rs)) return;
                 const uint32_t idb = __float_as_uint(sb.w) & ~kSnapOutlier;
                 const bool isA = id > idb;
-                sink(t, idb, isA, isA ? dx : -dx, isA ? dy : -dy, dist, rs);
+                const float mpart = have_m ? mp : k.src.mg[GATHER ? k.perm[t] : t].x;
+                sink(t, idb, isA, isA ? dx : -dx, isA ? dy : -dy, dist, rs, mpart);
             };
-            // Sorted-grid candidates.  With at most kFlatRows rows and 64 candidates (the
-            // common case) they form one flat list: a cheap pass marks the candidates within
-            // the squared contact distance in a 64-bit mask, and only marked ones take the
-            // exact test and the correction -- so a wave pays the correction path once per
-            // hit of its busiest lane, not once per candidate.
-            int32_t qcx0, qcx1, qcy0, qcy1;
-            const bool rect_ok = query_rect(g, xa.x - dpx, xa.y - dpy, ra + a.rmax + a.skin, qcx0, qcx1, qcy0, qcy1);
+            // Partner lists.  Two non-outliers are each within `skin` of their rebuild
+            // snapshot (after the common motion, which cancels between them), so bodies in
+            // contact now were within r_a + r_b + 2 skin at the rebuild: the first substep
+            // after a rebuild collects those partners from its (wider) grid walk, and the
+            // later substeps test only them.  Outlier partners come from the outlier table;
+            // an outlier, or a body whose list did not fit, walks the grid.
+            const float list_reach = 2.0f * a.skin + (1e-4f * (ra + a.rmax + 2.0f * a.skin) + 1e-5f);
+            const float Qc = ra + a.rmax + (GATHER ? list_reach : a.skin);
+            const bool use_list = !GATHER && nlen <= kNbrCap && !was_out;
+            // Sorted-grid candidates (first substep after a rebuild).  With at most kFlatRows
+            // rows and 64 candidates (the common case) they form one flat list: a cheap pass
+            // marks the candidates within the squared contact distance in a 64-bit mask, and
+            // only marked ones take the exact test and the correction -- so a wave pays the
+            // correction path once per hit of its busiest lane, not once per candidate.
+            int32_t qcx0 = 0, qcx1 = 0, qcy0 = 0, qcy1 = 0;
+            const bool rect_ok = GATHER && query_rect(g, xa.x - dpx, xa.y - dpy, Qc, qcx0, qcx1, qcy0, qcy1);
             uint32_t rlo[kFlatRows], rend[kFlatRows];  // slot range start, flat end of each row
             uint32_t n_flat = 0;
-            bool flat = rect_ok && qcy1 - qcy0 < kFlatRows;
+            bool flat = GATHER && rect_ok && qcy1 - qcy0 < kFlatRows;
             if (flat) {
 #pragma unroll
                 for (int r = 0; r < kFlatRows; ++r) {
@@ -213,7 +234,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     if (i >= rend[r - 1]) t = rlo[r] + (i - rend[r - 1]);
                 return t;
             };
-            uint64_t near = 0;  // flat candidates within the squared contact distance
+            const uint32_t nl = use_list ? nlen : 0u;
+            // slot of list entry i (< kNbrCap): the first eight from registers
+            auto list_slot = [&](uint32_t i) {
+                const uint32_t j = i >> 1;
+                const uint32_t a0 = (j & 1u) ? lw[1] : lw[0], a1 = (j & 1u) ? lw[3] : lw[2];
+                const uint32_t wd = j < 4u ? ((j & 2u) ? a1 : a0) : k.nbr[j * k.nb + s];
+                const int32_t dt = (int32_t)(int16_t)(uint16_t)((i & 1u) ? (wd >> 16) : (wd & 0xFFFFu));
+                const uint32_t t = (uint32_t)((int32_t)s + dt);
+                return t < k.nb ? t : k.nb - 1u;
+            };
+            // candidates (flat walk or partner list) within the squared contact distance
+            uint64_t near = 0;
+            uint64_t lst = 0;  // flat candidates within the list reach (first substep after a rebuild)
             if (flat) {
                 for (uint32_t f = 0; f < n_flat; f += 4) {
                     uint32_t tt[4];
@@ -227,24 +260,92 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     for (int u = 0; u < 4; ++u) {
                         const float dx = xa.x - sb[u].x, dy = xa.y - sb[u].y;
                         const float rs = ra + sb[u].z;
-                        const bool cand = f + u < n_flat && tt[u] != s && !(__float_as_uint(sb[u].w) & kSnapOutlier) &&
-                                          !(dx * dx + dy * dy > (rs * rs) * 1.00000095367431640625f);
+                        const float d2 = dx * dx + dy * dy;
+                        const bool base = f + u < n_flat && tt[u] != s && !(__float_as_uint(sb[u].w) & kSnapOutlier);
+                        const bool cand = base && !(d2 > (rs * rs) * 1.00000095367431640625f);
                         near |= cand ? (1ull << (f + u)) : 0ull;
+                        const float rl = rs + list_reach;
+                        lst |= (base && d2 <= rl * rl) ? (1ull << (f + u)) : 0ull;
+                    }
+                }
+            } else if (!GATHER) {
+#pragma unroll
+                for (uint32_t f = 0; f < kNbrCap; f += 4) {
+                    if (f < nl) {
+                        uint32_t tt[4];
+                        float4 sb[4];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; ++u) {
+                            tt[u] = f + u < nl ? list_slot(f + u) : s;
+                            sb[u] = k.snap_in[tt[u]];
+                        }
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; ++u) {
+                            const float dx = xa.x - sb[u].x, dy = xa.y - sb[u].y;
+                            const float rs = ra + sb[u].z;
+                            const bool cand = f + u < nl && !(__float_as_uint(sb[u].w) & kSnapOutlier) &&
+                                              !(dx * dx + dy * dy > (rs * rs) * 1.00000095367431640625f);
+                            near |= cand ? (1ull << (f + u)) : 0ull;
+                        }
                     }
                 }
             }
+            if (GATHER) {
+                // this body's partner list for the substeps up to the next rebuild
+                uint32_t n_l = flat ? (uint32_t)__popcll(lst) : (uint32_t)kNbrNone;
+                if (n_l > kNbrCap) n_l = kNbrNone;
+                if (n_l != kNbrNone) {
+                    uint32_t q = 0, word = 0;
+                    for (uint64_t m = lst; m != 0; m &= m - 1) {
+                        const int32_t dt = (int32_t)flat_slot((uint32_t)__builtin_ctzll(m)) - (int32_t)s;
+                        if (dt < -32768 || dt > 32767) {
+                            n_l = kNbrNone;
+                            break;
+                        }
+                        const uint32_t h16 = (uint32_t)(uint16_t)(int16_t)dt;
+                        if (q & 1u) {
+                            k.nbr[(q >> 1) * k.nb + s] = word | (h16 << 16);
+                        } else {
+                            word = h16;
+                        }
+                        ++q;
+                    }
+                    if (n_l != kNbrNone && (q & 1u)) k.nbr[(q >> 1) * k.nb + s] = word;
+                }
+                k.nbr_len[s] = (uint8_t)n_l;
+            }
             auto enumerate = [&](auto&& sink) {
-                if (flat) {
-                    for (uint64_t m = near; m != 0; m &= m - 1) {
-                        const uint32_t t = flat_slot((uint32_t)__builtin_ctzll(m));
-                        test(t, k.snap_in[t], sink);
+                if (flat || use_list) {
+                    // marked candidates four at a time: their snapshots and masses load together
+                    uint64_t m = near;
+                    while (m != 0) {
+                        uint32_t tt[kHitBatch];
+                        bool ok[kHitBatch];
+#pragma unroll
+                        for (int u = 0; u < kHitBatch; ++u) {
+                            ok[u] = m != 0;
+                            const uint32_t i = ok[u] ? (uint32_t)__builtin_ctzll(m) : 0u;
+                            m &= m - 1;
+                            tt[u] = ok[u] ? (GATHER ? flat_slot(i) : list_slot(i)) : s;
+                        }
+                        float4 sb[kHitBatch];
+                        float mp[kHitBatch];
+#pragma unroll
+                        for (int u = 0; u < kHitBatch; ++u) {
+                            sb[u] = k.snap_in[tt[u]];
+                            mp[u] = k.src.mg[GATHER ? k.perm[tt[u]] : tt[u]].x;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kHitBatch; ++u)
+                            if (ok[u]) test(tt[u], sb[u], mp[u], true, sink);
                     }
                 } else {
-                    grid_rows(g, cs, k.nb, xa.x - dpx, xa.y - dpy, ra + a.rmax + a.skin,
+                    grid_rows(g, cs, k.nb, xa.x - dpx, xa.y - dpy, Qc,
                               [&](uint32_t t0, uint32_t t1) {
                                   for (uint32_t t = t0; t < t1; ++t) {
                                       const float4 sb = k.snap_in[t];
-                                      if (t != s && !(__float_as_uint(sb.w) & kSnapOutlier)) test(t, sb, sink);
+                                      if (t != s && !(__float_as_uint(sb.w) & kSnapOutlier))
+                                          test(t, sb, 0.0f, false, sink);
                                   }
                               });
                 }
@@ -259,7 +360,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 if (!(R < 1e30f) || cx1 - cx0 > 64 || cy1 - cy0 > 64) {
                     for (uint32_t h = 0; h <= k.oh_mask; ++h)  // huge radius: every bucket (superset)
                         for (uint32_t t = k.oh_head[h]; t != kNil; t = k.oh_next[t])
-                            if (t != s && (!BATCHED || k.wid[t] == w)) test(t, k.snap_in[t], sink);
+                            if (t != s && (!BATCHED || k.wid[t] == w)) test(t, k.snap_in[t], 0.0f, false, sink);
                     return;
                 }
                 for (int32_t cy = cy0; cy <= cy1; ++cy)
@@ -274,7 +375,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                             if ((free_cell(sb.x, g.ox, g.inv_cell) >> kCoarseShift) != cx ||
                                 (free_cell(sb.y, g.oy, g.inv_cell) >> kCoarseShift) != cy)
                                 continue;
-                            test(t, sb, sink);
+                            test(t, sb, 0.0f, false, sink);
                         }
                     }
             };
@@ -282,7 +383,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
             // a push-out is folded into the same entry (order-dependent in the
             // reference, reproduced for the pair)
             auto correction = [&](bool first, auto&& put, uint32_t t, uint32_t idb, bool isA, float ax, float ay,
-                                  float dist, float rs) {
+                                  float dist, float rs, float mpart) {
                 float px = 0.0f, py = 0.0f;
                 const bool pushed = dist < 1e-6f;
                 if (pushed) {
@@ -301,7 +402,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     if (pushed) put(idb, px, py);
                     return;
                 }
-                const float mpart = k.src.mg[tp].x;
                 const float mA = isA ? mown : mpart, mB = isA ? mpart : mown;
                 const float ratio = 1.0f - mA / (mA + mB);
                 const float nx = ax / dist, ny = ay / dist;
@@ -325,17 +425,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
             };
             ContactList<kContacts, uint32_t> L;
             L.init();
-            enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs) {
+            enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs, float mp) {
                 if (isA) ++count;
                 correction(true, [&](uint32_t kk, float x, float y) { L.add(kk, x, y); }, t, idb, isA, ax, ay, dist,
-                           rs);
+                           rs, mp);
             });
             if (k.events && count > 0) {
                 // collision events of the pairs this body is A in, with both bodies'
                 // velocities before the contact pass (physics.rs:288-311)
                 uint32_t e = atomicAdd(&k.accum->event_count, count);
                 const float2 va = k.src.vel[g_own];
-                enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs) {
+                enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs, float) {
                     if (!isA) return;
                     if (e < k.event_cap) {
                         DevEvent ev;
@@ -356,7 +456,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 while (done < L.n) {  // more than kContacts: next key by selection, same order
                     uint32_t best = kNil;
                     float bx = 0.0f, by = 0.0f;
-                    enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs) {
+                    enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs,
+                                  float mp) {
                         correction(
                             false,
                             [&](uint32_t kk, float x, float y) {
@@ -366,7 +467,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                                     by = y;
                                 }
                             },
-                            t, idb, isA, ax, ay, dist, rs);
+                            t, idb, isA, ax, ay, dist, rs, mp);
                     });
                     if (best == kNil) break;
                     st.p.x = st.p.x + bx;
@@ -375,6 +476,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     ++done;
                 }
             }
+        } else if (GATHER) {
+            k.nbr_len[s] = kNbrNone;
         }
         // update_objects (physics.rs:326-358)
         st.rot = rw.x;
@@ -784,6 +887,8 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.event_cap = d.event_cap;
     // with events the old velocities stay readable: the new ones go to vel_next
     k.vel_out = k.events ? d.vel_next : k.dst.vel;
+    k.nbr = d.nbr;
+    k.nbr_len = d.nbr_len;
     const uint32_t blocks = div_up(d.nb, kThreads);
 #define BG_SPHERE(G, S, Z, C, P, B) k_sphere_substep<G, S, Z, C, P, B><<<blocks, kThreads, 0, d.stream>>>(k, a)
 #define BG_SPHERE_PB(G, S, Z, C)                 \

@@ -658,7 +658,7 @@ void World::free_arrays() {
     for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]);
     dfree(d.cell_count), dfree(d.perm);
     dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt), dfree(d.wstat), dfree(d.wvel);
-    dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list);
+    dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list), dfree(d.nbr), dfree(d.nbr_len);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
@@ -1001,6 +1001,9 @@ int World::upload() {
             HIP_TRY(dmalloc(&q.vreq, nb));
         }
         HIP_TRY(dmalloc(&d.snap_next, nb));
+        HIP_TRY(dmalloc(&d.nbr, (size_t)(kNbrCap / 2) * nb));
+        HIP_TRY(dmalloc(&d.nbr_len, nb));
+        HIP_TRY(hipMemsetAsync(d.nbr_len, kNbrNone, nb, s));
         d.oh_size = next_pow2(std::max<uint32_t>(1024u, nb / 4u));
         for (int i = 0; i < 3; ++i) {
             HIP_TRY(dmalloc(&d.oh_head[i], d.oh_size));
