@@ -78,6 +78,10 @@ struct DeviceWorld {
     uint32_t *oh_head[3] = {nullptr, nullptr, nullptr};
     uint32_t *oh_next[3] = {nullptr, nullptr, nullptr};
     uint32_t *oh_bits[3] = {nullptr, nullptr, nullptr};
+    // per wave of the substep launch, rotating with the tables: 1 if an outlier of the
+    // previous substep may touch one of the wave's bodies
+    uint32_t *oh_wave[3] = {nullptr, nullptr, nullptr};
+    uint32_t n_waves = 0;
     uint32_t oh_size = 0;  // power of two, >= 32
     // rebuild: bodies per cell, and the slot -> previous slot permutation the
     // first substep after a rebuild gathers its records through

@@ -63,6 +63,10 @@ struct SphereSubstep {
     uint32_t* oh_ins_bits;
     uint32_t* oh_clr_head;                 // table cleared for the substep after next
     uint32_t* oh_clr_bits;
+    const uint32_t* __restrict__ oh_wave;  // per wave: an outlier may touch its bodies this substep
+    uint32_t* oh_ins_wave;                 // the same for the next substep
+    uint32_t* oh_clr_wave;
+    uint32_t n_waves;
     uint32_t oh_mask;
     uint32_t nb;
     StepAccum* accum;
@@ -133,6 +137,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
         k.oh_clr_head[i] = kNil;
         if ((i & 31u) == 0u) k.oh_clr_bits[i >> 5] = 0u;
     }
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < k.n_waves; i += gridDim.x * kThreads)
+        k.oh_clr_wave[i] = 0u;
     // world of this slot (lanes past the end take the last body's world)
     const uint32_t w = BATCHED ? k.wid[s < k.nb ? s : k.nb - 1] : 0u;
     uint32_t new_outliers = 0, count = 0, pushes = 0, flags = 0;
@@ -152,6 +158,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
         const float2 rw = k.src.rw[g_own];
         const float2 x0 = k.x0[s];
         const uint32_t nlen = GATHER ? (uint32_t)kNbrNone : (uint32_t)k.nbr_len[s];
+        // an outlier of the previous substep may touch a body of this wave
+        const uint32_t near_out = k.oh_wave[__builtin_amdgcn_readfirstlane(s / BG_WAVE)];
         // the first eight partner-list entries, loaded with the record (the rest on demand)
         uint32_t lw[4];
 #pragma unroll
@@ -349,7 +357,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
                                   }
                               });
                 }
-                if (nout == 0u) return;
+                // A non-outlier skips the table unless its wave was flagged: every outlier
+                // flags the waves holding bodies it could touch (see its insertion below).
+                if (nout == 0u || (near_out == 0u && !was_out)) return;
                 // outlier table: coarse cells of this world around the current snapshot
                 float R = ra + a.rmax;
                 R = R + (1e-5f * (fabsf(xa.x) + fabsf(xa.y) + R) + 1e-6f);
@@ -543,6 +553,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
             const uint32_t old = atomicExch(&k.oh_ins_head[h], s);
             k.oh_ins_next[s] = old;
             atomicOr(&k.oh_ins_bits[h >> 5], 1u << (h & 31u));
+            // Flag the waves whose bodies this outlier could touch next substep.  A body
+            // in contact is within 2 r_max of it; a non-outlier is within `skin` of its
+            // rebuild position plus the world's common motion, so its rebuild cell lies
+            // within 2 r_max + skin of this position with that motion removed (the skin
+            // test's expression) -- and the grid holds bodies by rebuild cell.
+            const float2 vw = k.wvel[w];
+            const float ddx = a.drift_x + vw.x * a.t_cur, ddy = a.drift_y + vw.y * a.t_cur;
+            int32_t fx0, fx1, fy0, fy1;
+            query_rect(g, x.x - ddx, x.y - ddy, 2.0f * a.rmax + a.skin, fx0, fx1, fy0, fy1);
+            const uint32_t* __restrict__ csw = k.cell_start + k.cell_base[w];
+            for (int32_t cy = fy0; cy <= fy1; ++cy) {
+                const uint32_t rowc = (uint32_t)cy * (uint32_t)g.nx;
+                const uint32_t t0 = csw[rowc + (uint32_t)fx0];
+                uint32_t t1 = csw[rowc + (uint32_t)fx1 + 1u];
+                t1 = t1 < k.nb ? t1 : k.nb;
+                for (uint32_t wv = t0 / BG_WAVE; t0 < t1 && wv <= (t1 - 1u) / BG_WAVE; ++wv) k.oh_ins_wave[wv] = 1u;
+            }
         }
         if (CONSTRAINTS) apply_constraints(st.p, k.con_pos, k.con_r, k.nk);
         k.dst.pp[s] = make_float4(st.p.x, st.p.y, st.po.x, st.po.y);
@@ -755,6 +782,8 @@ struct SphereScatter {
     uint32_t total_cells;
     uint32_t* heads[3];
     uint32_t* bits[3];
+    uint32_t* waves[3];
+    uint32_t n_waves;
     uint32_t oh_size;
     uint32_t nb;
     float* wstat;
@@ -774,6 +803,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
         k.heads[t][e] = kNil;
         if ((e & 31u) == 0u) k.bits[t][e >> 5] = 0u;
     }
+    for (uint32_t h = i; h < 3u * k.n_waves; h += stride) k.waves[h / k.n_waves][h % k.n_waves] = 0u;
     if (i >= k.nb) return;
     const uint32_t slot = k.cell_start[k.key[i]] + k.rank[i];
     if (slot >= k.nb) return;  // consistent counts and scan never get here
@@ -833,8 +863,10 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     for (int t = 0; t < 3; ++t) {
         k.heads[t] = d.oh_head[t];
         k.bits[t] = d.oh_bits[t];
+        k.waves[t] = d.oh_wave[t];
     }
     k.oh_size = d.oh_size;
+    k.n_waves = d.n_waves;
     k.nb = d.nb;
     k.wstat = d.wstat;
     k.n_wstat = 8u * d.stat_rep * d.n_worlds;
@@ -876,6 +908,10 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.oh_ins_bits = d.oh_bits[(j + 1) % 3];
     k.oh_clr_head = d.oh_head[(j + 2) % 3];
     k.oh_clr_bits = d.oh_bits[(j + 2) % 3];
+    k.oh_wave = d.oh_wave[j % 3];
+    k.oh_ins_wave = d.oh_wave[(j + 1) % 3];
+    k.oh_clr_wave = d.oh_wave[(j + 2) % 3];
+    k.n_waves = d.n_waves;
     k.oh_mask = d.oh_size - 1;
     k.nb = d.nb;
     k.accum = d.accum;

@@ -655,7 +655,7 @@ void World::free_arrays() {
         dfree(q->pp), dfree(q->snap), dfree(q->vel), dfree(q->rw), dfree(q->mg), dfree(q->bflags), dfree(q->off);
         dfree(q->groups), dfree(q->inertia), dfree(q->torque), dfree(q->acc), dfree(q->vreq);
     }
-    for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]);
+    for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]), dfree(d.oh_wave[i]);
     dfree(d.cell_count), dfree(d.perm);
     dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt), dfree(d.wstat), dfree(d.wvel);
     dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list), dfree(d.nbr), dfree(d.nbr_len);
@@ -1009,6 +1009,9 @@ int World::upload() {
             HIP_TRY(dmalloc(&d.oh_head[i], d.oh_size));
             HIP_TRY(dmalloc(&d.oh_next[i], nb));
             HIP_TRY(dmalloc(&d.oh_bits[i], d.oh_size / 32u));
+            d.n_waves = (nb + 255u) / 256u * 4u;  // waves of the 256-thread substep launch
+            HIP_TRY(dmalloc(&d.oh_wave[i], d.n_waves));
+            HIP_TRY(hipMemsetAsync(d.oh_wave[i], 0, sizeof(uint32_t) * d.n_waves, s));
         }
         HIP_TRY(dmalloc(&d.cell_count, (size_t)d.total_cells + 1));
         HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.total_cells + 1), s));
