@@ -1,0 +1,72 @@
+"""Partner-list fallbacks of the fused sphere step, bit-exact against the oracle.
+
+Between rebuilds a body tests only the partners its list recorded; a body
+whose list would not fit (more than 16 partners within r_a + r_b + 2 skin,
+or a partner more than 32767 slots away in the sorted grid) walks the grid
+instead.  Both fallbacks must leave the results untouched.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from blobs_amd import scenes
+from oracle.oracle import OracleWorld
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def list_lengths(p):
+    """Histogram of the device's partner-list lengths: [0..16], then bodies
+    without a list (diagnostic hook of the library, not part of the ABI)."""
+    fn = p._lib.bgx_list_lengths
+    fn.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    h = (C.c_uint32 * 18)()
+    assert fn(p._w, h) == 0
+    return np.array(h[:], np.int64)
+
+
+def run_both(sc, steps, broadphase):
+    o = OracleWorld(sc, np.float32)
+    o.step(sc.dt, steps)
+    p, _ = sc.build_physics()
+    p.set_broadphase(*broadphase)
+    p.step(sc.dt, steps)
+    assert p.step_path() == "sphere"
+    return p, o
+
+
+@pytest.mark.parametrize("broadphase", [(2.0, 3, 0.5), (1.0, 6, 0.5), (1.0, 2, 0.05)])
+def test_dense_scene_long_and_short_lists(broadphase):
+    """Packing 0.4 with a wide reach overflows most lists (more than 16
+    partners); a tiny skin makes many outliers, which never use their lists."""
+    sc = scenes.cfg2_plain_large(n=4096, steps=6, seed=32)
+    p, o = run_both(sc, 6, broadphase)
+    bad = np.argwhere(bits(p.bodies()["position"]) != bits(o.pos))
+    assert bad.shape[0] == 0, f"{bad.shape[0]} coordinates differ"
+    assert p.contact_count() == o.contact_count > 500
+    h = list_lengths(p)
+    assert h.sum() == sc.n_bodies
+    if broadphase[0] == 2.0:
+        assert h[17] > sc.n_bodies // 2, f"the wide reach should overflow most lists: {h}"
+    else:
+        assert h[1:17].sum() > sc.n_bodies // 2, f"most bodies should have a list: {h}"
+
+
+def test_long_grid_rows_overflow_slot_deltas():
+    """A 73,600 x 10 strip of 160,000 spheres: about 40,000 bodies per grid
+    row, so partners in the next row lie more than 32767 slots away and
+    those bodies walk the grid."""
+    n, w, h = 160000, 73600.0, 10.0
+    pos, vel, rad, _, _ = scenes.spheres(n, 0.5, 1.0, (0.0, 0.0, w, h), 2.0, 31)
+    sc = scenes._single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1 / 120, 3, {})
+    p, o = run_both(sc, 3, (1.25, 3, 0.25))
+    bad = np.argwhere(bits(p.bodies()["position"]) != bits(o.pos))
+    assert bad.shape[0] == 0, f"{bad.shape[0]} coordinates differ"
+    assert p.contact_count() == o.contact_count > 10000
+    h = list_lengths(p)
+    assert h[17] > 1000 and h[1:17].sum() > 1000, f"both lists and far-partner fallbacks expected: {h}"
