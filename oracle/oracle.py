@@ -25,8 +25,9 @@ LIB = HERE / "liboracle.so"
 _lib = None
 
 
-def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+def build(force: bool = False) -> Path:
+    """Compile liboracle.so (make; `force` recompiles even when it is up to date)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)] + (["-B"] if force else []), check=True)
     return LIB
 
 
