@@ -96,7 +96,7 @@ def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) 
     from blobs_amd import scenes
     from blobs_amd.physics import Physics
     t0 = time.perf_counter()
-    scs = scenes.cfg4_scenes(n_worlds)
+    scs = scenes.uneven_batch_scenes(n_worlds)
     worlds = [sc.build_physics()[0] for sc in scs]
     batch = Physics.batch(worlds)
     n_bodies = batch.rbd_count()
@@ -136,7 +136,7 @@ def differentiable_leg(n_worlds: int, n_steps: int, ck_every: int) -> dict:
     from blobs_amd import scenes
     from blobs_amd.rollout import Rollout
     t0 = time.perf_counter()
-    ws, targets = scenes.cfg5_scenes(n_worlds)
+    ws, targets = scenes.rollout_scenes(n_worlds)
     r = Rollout([w.build_physics()[0] for w in ws], targets, checkpoint_every=ck_every)
     log(f"differentiable: {n_worlds} worlds built in {time.perf_counter() - t0:.1f} s")
     dt = ws[0].dt
@@ -173,7 +173,7 @@ def multi_collider_leg(steps: int = 20, warmup: int = 3) -> dict:
     """The multi-collider scene on the general path: 65,536 bodies with 1-8
     sphere colliders each, 32,768 springs, 16,384 fixed joints, step(1/60)."""
     from blobs_amd import scenes
-    sc = scenes.cfg3_multi()
+    sc = scenes.multi_collider_scene()
     p, _ = sc.build_physics()
     p.step(sc.dt, warmup)
     p._lib.bg_synchronize(p._w)
@@ -222,7 +222,7 @@ def main() -> int:
     from blobs_amd import scenes
 
     t0 = time.perf_counter()
-    sc = scenes.cfg2_plain_large(n=args.n, steps=args.steps)
+    sc = scenes.plain_large_scene(n=args.n, steps=args.steps)
     log(f"scene: {sc.n_bodies} spheres, box {sc.meta['box_side']:.1f}, forced overlaps "
         f"{sc.meta['forced_overlaps']} ({time.perf_counter() - t0:.1f} s)")
     tuned = None

@@ -194,7 +194,7 @@ def box_side_for_packing(n, rmin, rmax, phi):
     return math.sqrt(n * math.pi * er2 / phi)
 
 
-def cfg1_plain_small(n=1024, steps=600, walls=True, seed=1):
+def plain_small_scene(n=1024, steps=600, walls=True, seed=1):
     """1,024 spheres r~U[0.1,0.3] in a 64x64 box, v~N(0,1), g=(0,-9.81),
     dt=1/60.  Walls: static spheres r=0.5 one unit apart just outside the box,
     mass_override 1e30 so contacts leave them in place (blobs has no plane
@@ -223,7 +223,7 @@ def cfg1_plain_small(n=1024, steps=600, walls=True, seed=1):
     return _single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1.0 / 60.0, steps, meta)
 
 
-def cfg2_plain_large(n=1 << 20, steps=100, seed=2, phi=0.4):
+def plain_large_scene(n=1 << 20, steps=100, seed=2, phi=0.4):
     """n spheres r~U[0.5,1.0], square box at packing fraction phi, centres U
     with overlap rejection, v~N(0,2), gravity on, dt=1/120, no walls."""
     side = box_side_for_packing(n, 0.5, 1.0, phi)
@@ -232,7 +232,7 @@ def cfg2_plain_large(n=1 << 20, steps=100, seed=2, phi=0.4):
     return _single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1.0 / 120.0, steps, meta)
 
 
-def cfg3_multi(n=65536, n_springs=32768, n_joints=16384, steps=300, seed=3, phi=0.2):
+def multi_collider_scene(n=65536, n_springs=32768, n_joints=16384, steps=300, seed=3, phi=0.2):
     """n bodies with U{1..8} sphere colliders (offsets uniform in a disc of
     radius 2, collider r~U[0.2,0.5]), n_springs springs and n_joints fixed
     joints between random bodies within distance 6 (rest = initial distance,
@@ -286,7 +286,7 @@ def batch_world_sizes(n_worlds=4096, lo=64, hi=16384, seed=4):
     return np.rint(rng.uniform(n_worlds, lo, hi, log=True)).astype(np.int64), rng
 
 
-def cfg4_scenes(n_worlds=4096, lo=64, hi=16384, seed=4, size_cap=None):
+def uneven_batch_scenes(n_worlds=4096, lo=64, hi=16384, seed=4, size_cap=None):
     """The batched scene: n_worlds single-sphere worlds, N_w log-uniform in
     [lo, hi], packing U[0.2, 0.5] per world, all drawn from one seeded
     stream (sizes, then packings, then each world's bodies)."""
@@ -294,10 +294,10 @@ def cfg4_scenes(n_worlds=4096, lo=64, hi=16384, seed=4, size_cap=None):
     packs = rng.uniform(n_worlds, 0.2, 0.5)
     if size_cap is not None:
         sizes = np.minimum(sizes, size_cap)
-    return [cfg4_world(int(sizes[k]), float(packs[k]), rng) for k in range(n_worlds)]
+    return [uneven_batch_world(int(sizes[k]), float(packs[k]), rng) for k in range(n_worlds)]
 
 
-def cfg4_world(size, packing, seed_rng: Rng, vel_sigma=1.0):
+def uneven_batch_world(size, packing, seed_rng: Rng, vel_sigma=1.0):
     """One batched world: single spheres r~U[0.1,0.5] at the given packing."""
     side = box_side_for_packing(int(size), 0.1, 0.5, float(packing))
     pos = np.zeros((size, 2), np.float32)
@@ -309,7 +309,7 @@ def cfg4_world(size, packing, seed_rng: Rng, vel_sigma=1.0):
                                 {"config": "batched_world", "box_side": side, "forced_overlaps": int(forced)})
 
 
-def cfg5_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0, kmax=1e3):
+def rollout_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0, kmax=1e3):
     """One differentiable-rollout world: n single spheres r~U[0.5,1.0] at the
     given packing (square box centred on the origin: position Verlet forms
     velocities as p - p_old in f32, so small coordinates keep them
@@ -338,12 +338,12 @@ def cfg5_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0, km
     return sc
 
 
-def cfg5_scenes(n_worlds=256, n=2048, n_springs=1024, seed=5, **kw):
+def rollout_scenes(n_worlds=256, n=2048, n_springs=1024, seed=5, **kw):
     """The differentiable rollout: n_worlds worlds of n spheres + n_springs
     springs, drawn from one seeded stream, plus one seeded target point per
     world (uniform over the world's box)."""
     rng = Rng(seed)
-    worlds = [cfg5_world(n, n_springs, rng, **kw) for _ in range(n_worlds)]
+    worlds = [rollout_world(n, n_springs, rng, **kw) for _ in range(n_worlds)]
     side = np.array([w.meta["box_side"] for w in worlds], np.float32)
     u = rng.uniform(2 * n_worlds, -0.5, 0.5).reshape(n_worlds, 2)
     targets = (u * side[:, None]).astype(np.float32)

@@ -69,7 +69,7 @@ def test_physics_new_defaults():
 def test_insert_collider_doubles_mass_and_matches_oracle():
     """Q2 (collider.rs:179-181 + physics.rs:144) through the product's
     construction path equals the oracle's restatement bit for bit."""
-    sc = scenes.cfg3_multi(n=256, n_springs=64, n_joints=32, steps=1)
+    sc = scenes.multi_collider_scene(n=256, n_springs=64, n_joints=32, steps=1)
     p, bh = sc.build_physics()
     m, I, com = oracle.mass_setup(sc)
     for b in range(sc.n_bodies):
@@ -80,7 +80,7 @@ def test_insert_collider_doubles_mass_and_matches_oracle():
 
 
 def test_create_fixed_joint_distance_and_target_match_oracle():
-    sc = scenes.cfg3_multi(n=256, n_springs=8, n_joints=64, steps=1)
+    sc = scenes.multi_collider_scene(n=256, n_springs=8, n_joints=64, steps=1)
     rot = np.linspace(-1, 1, sc.n_bodies).astype(np.float32)
     sc.rotation = rot
     sc.joint_anchor_a = np.full((64, 2), 0.25, np.float32)
@@ -186,7 +186,7 @@ def test_rigid_body_builder_defaults():
 def test_batch_construction_on_host():
     """bg_batch_create copies single-sphere worlds (host only) and refuses
     worlds that are not, or whose parameters differ."""
-    scs = scenes.cfg4_scenes(3, lo=64, hi=200, seed=48)
+    scs = scenes.uneven_batch_scenes(3, lo=64, hi=200, seed=48)
     src = [sc.build_physics()[0] for sc in scs]
     b = Physics.batch(src)
     assert b.batch_size() == 3
@@ -213,7 +213,7 @@ def test_batch_construction_on_host():
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_batch_run_without_device_fails_loudly():
-    sc = scenes.cfg4_scenes(2, lo=64, hi=100, seed=49)
+    sc = scenes.uneven_batch_scenes(2, lo=64, hi=100, seed=49)
     b = Physics.batch([s.build_physics()[0] for s in sc])
     with pytest.raises(L.BlobsError) as e:
         b.run_chunks(1 / 60, 10, 2)

@@ -16,7 +16,7 @@ torch = pytest.importorskip("torch")
 
 def small(n_worlds=2, n=48, n_springs=24, seed=55, packing=0.35):
     rng = scenes.Rng(seed)
-    ws = [scenes.cfg5_world(n, n_springs, rng, packing=packing) for _ in range(n_worlds)]
+    ws = [scenes.rollout_world(n, n_springs, rng, packing=packing) for _ in range(n_worlds)]
     targets = np.array([[w.meta["box_side"] * 0.3, w.meta["box_side"] * 0.6] for w in ws], np.float32)
     return ws, targets
 

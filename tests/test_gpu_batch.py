@@ -39,7 +39,7 @@ def assert_state_equal(p, q, tag):
 
 
 def small_batch(n_worlds=12, seed=44, cap=1500):
-    return scenes.cfg4_scenes(n_worlds, lo=64, hi=4096, seed=seed, size_cap=cap)
+    return scenes.uneven_batch_scenes(n_worlds, lo=64, hi=4096, seed=seed, size_cap=cap)
 
 
 def test_batch_matches_worlds_stepped_alone():
@@ -96,7 +96,7 @@ def test_batch_sub_world_q1_old_dt_is_per_world():
 def test_chunks_equal_steps(chunk, n_chunks, interval):
     """Graph replay (cyclic or not, any remainder) leaves the same state as
     bg_step, and stepping on afterwards stays identical."""
-    sc = scenes.cfg1_plain_small(steps=100)
+    sc = scenes.plain_small_scene(steps=100)
     p = sc.build_physics()[0]
     q = sc.build_physics()[0]
     for w in (p, q):
@@ -113,7 +113,7 @@ def test_chunks_equal_steps(chunk, n_chunks, interval):
 
 def test_batch_chunks_1m_scale_small_worlds():
     """A batch of many tiny worlds (several per wave) through graph chunks."""
-    scs = scenes.cfg4_scenes(64, lo=64, hi=128, seed=47)
+    scs = scenes.uneven_batch_scenes(64, lo=64, hi=128, seed=47)
     src = [sc.build_physics()[0] for sc in scs]
     batch = Physics.batch(src)
     batch.run_chunks(scs[0].dt, 5, 2)
@@ -125,7 +125,7 @@ def test_batch_chunks_1m_scale_small_worlds():
 
 
 def test_autotune_keeps_results():
-    sc = scenes.cfg1_plain_small(steps=100)
+    sc = scenes.plain_small_scene(steps=100)
     p = sc.build_physics()[0]
     cs, sk, iv = p.autotune(sc.dt, 2)
     assert cs >= 0.25 and 0.0 < sk <= 4.0 and iv >= 1

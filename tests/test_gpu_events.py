@@ -55,7 +55,7 @@ def assert_same_events(hs, vs, cols, vels):
 
 @pytest.mark.parametrize("broadphase", [None, (1.25, 3, 0.25)])
 def test_sphere_path_events_match_oracle(broadphase):
-    sc = scenes.cfg1_plain_small(n=1024, steps=30, seed=11)
+    sc = scenes.plain_small_scene(n=1024, steps=30, seed=11)
     _, cols, vels = oracle_events(sc, 30)
     p, hs, vs, dropped = gpu_events(sc, 30, broadphase=broadphase)
     assert p.step_path() == "sphere"
@@ -68,7 +68,7 @@ def test_sphere_path_events_match_oracle(broadphase):
 def test_general_path_events_match_oracle():
     # multi-collider bodies and springs; no joints, whose rotations differ from
     # the oracle's in the last bit of sinf/cosf/atan2f and would fork the trajectory
-    sc = scenes.cfg3_multi(n=1024, n_springs=512, n_joints=0, steps=10, seed=12, phi=0.3)
+    sc = scenes.multi_collider_scene(n=1024, n_springs=512, n_joints=0, steps=10, seed=12, phi=0.3)
     _, cols, vels = oracle_events(sc, 10)
     p, hs, vs, dropped = gpu_events(sc, 10)
     assert p.step_path() == "general"
@@ -78,7 +78,7 @@ def test_general_path_events_match_oracle():
 
 
 def test_events_overflow_are_counted_and_kept_events_are_the_reference_s():
-    sc = scenes.cfg2_plain_large(n=4096, steps=5, seed=13)
+    sc = scenes.plain_large_scene(n=4096, steps=5, seed=13)
     _, cols, vels = oracle_events(sc, 5)
     cap = 400
     assert cols.shape[0] > 2 * cap
@@ -99,7 +99,7 @@ def test_events_overflow_are_counted_and_kept_events_are_the_reference_s():
 
 
 def test_partial_drains_and_reupload_keep_the_stream():
-    sc = scenes.cfg2_plain_large(n=4096, steps=6, seed=14)
+    sc = scenes.plain_large_scene(n=4096, steps=6, seed=14)
     _, cols, vels = oracle_events(sc, 6)
     p, _ = sc.build_physics()
     p.set_event_capacity(1 << 20)
@@ -114,7 +114,7 @@ def test_partial_drains_and_reupload_keep_the_stream():
 
 
 def test_events_off_by_default_and_after_zero_capacity():
-    sc = scenes.cfg2_plain_large(n=1024, steps=5, seed=15)
+    sc = scenes.plain_large_scene(n=1024, steps=5, seed=15)
     p, _ = sc.build_physics()
     p.step(sc.dt, 5)
     hs, _, d = p.drain_events()
@@ -128,7 +128,7 @@ def test_events_off_by_default_and_after_zero_capacity():
 
 
 def test_batched_chunks_with_events_match_plain_steps():
-    ws = scenes.cfg4_scenes(n_worlds=6, lo=64, hi=512, seed=16)
+    ws = scenes.uneven_batch_scenes(n_worlds=6, lo=64, hi=512, seed=16)
     steps = 6
     # each world alone, events drained per world
     alone = []
@@ -148,7 +148,7 @@ def test_batched_chunks_with_events_match_plain_steps():
 
 def test_debug_data_after_steps():
     """Physics::debug_data (debug.rs:34-91) from the stepped state."""
-    sc = scenes.cfg3_multi(n=256, n_springs=64, n_joints=32, steps=5, seed=17, phi=0.2)
+    sc = scenes.multi_collider_scene(n=256, n_springs=64, n_joints=32, steps=5, seed=17, phi=0.2)
     p, bh = sc.build_physics()
     p.step(sc.dt, 5)
     dd = p.debug_data()

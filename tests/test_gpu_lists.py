@@ -44,7 +44,7 @@ def run_both(sc, steps, broadphase):
 def test_dense_scene_long_and_short_lists(broadphase):
     """Packing 0.4 with a wide reach overflows most lists (more than 16
     partners); a tiny skin makes many outliers, which never use their lists."""
-    sc = scenes.cfg2_plain_large(n=4096, steps=6, seed=32)
+    sc = scenes.plain_large_scene(n=4096, steps=6, seed=32)
     p, o = run_both(sc, 6, broadphase)
     bad = np.argwhere(bits(p.bodies()["position"]) != bits(o.pos))
     assert bad.shape[0] == 0, f"{bad.shape[0]} coordinates differ"

@@ -36,11 +36,11 @@ def assert_bitexact(got, want, what):
 
 
 def sphere_scene(seed=21, n=2048):
-    return scenes.cfg2_plain_large(n=n, steps=8, seed=seed)
+    return scenes.plain_large_scene(n=n, steps=8, seed=seed)
 
 
 def general_scene(seed=22):
-    return scenes.cfg3_multi(n=1024, n_springs=512, n_joints=0, steps=8, seed=seed, phi=0.3)
+    return scenes.multi_collider_scene(n=1024, n_springs=512, n_joints=0, steps=8, seed=seed, phi=0.3)
 
 
 @pytest.mark.parametrize("kind", ["sphere", "general"])

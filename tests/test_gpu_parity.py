@@ -46,7 +46,7 @@ def compare_state(p, o, tag):
 
 def test_integration_only_bitexact_600_steps():
     """Gravity + Verlet + velocity requests (Q1 included), 600 x step(1/60)."""
-    sc = scenes.cfg1_plain_small(steps=600)
+    sc = scenes.plain_small_scene(steps=600)
     p = gpu_world(sc, collisions_enabled=False)
     o = OracleWorld(sc, np.float32, collisions_enabled=False)
     p.step(sc.dt, 600)
@@ -58,7 +58,7 @@ def test_integration_only_bitexact_600_steps():
 @pytest.mark.parametrize("steps", [1, 30])
 def test_plain_small_contacts_bitexact(steps):
     """1,024 spheres + static walls, contacts on: positions and contact count bit-exact."""
-    sc = scenes.cfg1_plain_small(steps=steps)
+    sc = scenes.plain_small_scene(steps=steps)
     p = gpu_world(sc)
     o = OracleWorld(sc, np.float32)
     for k in range(steps):
@@ -72,7 +72,7 @@ def test_plain_small_contacts_bitexact(steps):
 def test_general_path_equals_sphere_path():
     """A zero spring forces the general (multi-kernel) path; it adds +0 to
     accelerations, so both device paths must agree bit for bit."""
-    sc = scenes.cfg1_plain_small(steps=20)
+    sc = scenes.plain_small_scene(steps=20)
     p1 = gpu_world(sc)
     p2, bh = sc.build_physics()
     p2.springs.insert(Spring(int(bh[0]), int(bh[1]), 0.0, 0.0, 0.0))
@@ -87,7 +87,7 @@ def test_general_path_equals_sphere_path():
 def test_broadphase_settings_do_not_change_results(interval, scale, skin):
     """Rebuild interval, cell size and skin (tiny skins force many outliers)
     change only speed: positions and contact counts stay bit-exact."""
-    sc = scenes.cfg1_plain_small(steps=12)
+    sc = scenes.plain_small_scene(steps=12)
     o = OracleWorld(sc, np.float32)
     o.step(sc.dt, 12)
     p = gpu_world(sc)
@@ -100,7 +100,7 @@ def test_broadphase_settings_do_not_change_results(interval, scale, skin):
 def test_plain_large_one_step_bitexact_1m():
     """1,048,576 spheres at packing 0.4: one step(1/120) bit-exact vs the
     oracle (grid-enumerated, same (i, j) visit order as the O(C^2) loop)."""
-    sc = scenes.cfg2_plain_large(steps=1)
+    sc = scenes.plain_large_scene(steps=1)
     p = gpu_world(sc)
     o = OracleWorld(sc, np.float32, use_grid=True)
     p.step(sc.dt)
@@ -120,7 +120,7 @@ def test_plain_large_invariants_100_steps():
     for every setting), positions stay finite, and along x (no gravity) the
     mass-weighted centroid follows the initial velocities, because contacts
     conserve sum(m x)."""
-    sc = scenes.cfg2_plain_large(steps=100)
+    sc = scenes.plain_large_scene(steps=100)
     p1 = gpu_world(sc)
     p2 = gpu_world(sc)
     p2.set_broadphase(1.5, 1, 0.05)
@@ -144,7 +144,7 @@ def test_plain_large_sphere_vs_general_path_20_steps():
     """At 1M the fused sphere kernel (cell-ordered bodies) and the general
     multi-kernel path (arena-ordered bodies, sorted collider index) are two
     independent device implementations of the same step: bit-identical."""
-    sc = scenes.cfg2_plain_large(steps=20)
+    sc = scenes.plain_large_scene(steps=20)
     p1 = gpu_world(sc)
     p2, bh = sc.build_physics()
     p2.springs.insert(Spring(int(bh[0]), int(bh[1]), 0.0, 0.0, 0.0))
@@ -163,7 +163,7 @@ def test_multi_collider_springs_joints_tolerance():
     is the last bit of glibc's atan2f/sinf/cosf (not correctly rounded; the
     device evaluates in f64 and rounds).  Bar: max |dx| <= 1e-4 * (smallest
     collider radius) after one step; bodies that never rotate are exact."""
-    sc = scenes.cfg3_multi(n=4096, n_springs=2048, n_joints=1024, steps=10)
+    sc = scenes.multi_collider_scene(n=4096, n_springs=2048, n_joints=1024, steps=10)
     p = gpu_world(sc)
     o = OracleWorld(sc, np.float32)
     p.step(sc.dt)

@@ -32,7 +32,7 @@ torch = pytest.importorskip("torch")
 
 def worlds(n_worlds=2, n=64, n_springs=32, seed=57, packing=0.3):
     rng = scenes.Rng(seed)
-    ws = [scenes.cfg5_world(n, n_springs, rng, packing=packing) for _ in range(n_worlds)]
+    ws = [scenes.rollout_world(n, n_springs, rng, packing=packing) for _ in range(n_worlds)]
     targets = np.array([[w.meta["box_side"] * 0.25, w.meta["box_side"] * 0.75] for w in ws], np.float32)
     return ws, targets
 

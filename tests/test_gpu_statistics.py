@@ -38,7 +38,7 @@ def pile_stats(pos, vel, radius, mass, dyn):
 
 
 def test_penetration_and_energy_statistics_over_1000_steps_match_fp64():
-    sc = scenes.cfg1_plain_small(n=1024, steps=1000, seed=1)
+    sc = scenes.plain_small_scene(n=1024, steps=1000, seed=1)
     p, _ = sc.build_physics()
     o = OracleWorld(sc, np.float64)
     mass = mass_setup(sc)[0].astype(np.float64)

@@ -169,7 +169,7 @@ def test_contact_pair_set_matches_bruteforce(use_grid):
 def test_grid_enumeration_equals_reference_loop():
     """The grid variant (used for large scenes) visits the same (i, j)
     sequence as the O(C^2) loop: trajectories are bit-identical."""
-    for sc in (scenes.cfg1_plain_small(n=400, steps=1), _random_multi_scene(11)):
+    for sc in (scenes.plain_small_scene(n=400, steps=1), _random_multi_scene(11)):
         a = OracleWorld(sc, np.float32, use_grid=False)
         b = OracleWorld(sc, np.float32, use_grid=True)
         a.step(1 / 60, 15)
@@ -344,7 +344,7 @@ def test_mass_setup_double_push_closed_form():
 # --------------------------------------------------------------- twins
 
 def test_f32_restatement_tracks_fp64_twin():
-    sc = scenes.cfg1_plain_small(n=512, steps=2)
+    sc = scenes.plain_small_scene(n=512, steps=2)
     a = OracleWorld(sc, np.float32)
     b = OracleWorld(sc, np.float64)
     a.step(sc.dt, 2)
