@@ -300,11 +300,14 @@ int bgx_world_accum(bg_world* w, uint32_t* out4) {
 }
 
 // Internal diagnostic hook: histogram of partner-list lengths (fused sphere
-// path), [0..16] by length, [17] bodies that walk the grid.
-int bgx_list_lengths(bg_world* w, uint32_t* out18) {
-    if (!out18) return BG_ERR_INVALID_ARGUMENT;
-    return guarded(w, [&](bg::World& x) { return x.debug_list_lengths(out18); });
+// path): out[0..cap] by length, out[cap + 1] bodies that walk the grid; the
+// caller passes cap + 2 entries and learns cap from bgx_list_capacity.
+int bgx_list_lengths(bg_world* w, uint32_t* out) {
+    if (!out) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) { return x.debug_list_lengths(out); });
 }
+
+uint32_t bgx_list_capacity(void) { return bg::kNbrCap; }
 
 int bg_rollout_create(bg_world* const* worlds, uint32_t n_worlds, const float* targets_xy, uint32_t checkpoint_every,
                       bg_rollout** out) {

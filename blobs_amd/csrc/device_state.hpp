@@ -50,7 +50,7 @@ struct SphereArrays {
 
 constexpr uint32_t kSnapOutlier = 0x80000000u;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
-constexpr uint32_t kNbrCap = 16;      // partner-list entries per body (sphere path)
+constexpr uint32_t kNbrCap = 32;      // partner-list entries per body (sphere path)
 constexpr uint8_t kNbrNone = 0xFF;    // no list: the body walks the grid
 
 struct DeviceWorld {

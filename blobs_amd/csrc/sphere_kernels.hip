@@ -37,7 +37,8 @@ namespace bg {
 namespace {
 
 constexpr int kCoarseShift = 2;  // outlier table cells are 4x4 grid cells
-constexpr int kFlatRows = 6;      // grid rows a query walks as one flat candidate list (<= 64 candidates)
+constexpr int kFlatRows = 6;      // grid rows a query walks as one flat candidate list
+constexpr uint32_t kFlatMax = 128;  // candidates of a flat list (two 64-bit masks)
 constexpr int kContacts = 4;      // contacts summed from registers; more take a selection pass in key order
 constexpr int kHitBatch = 2;      // marked candidates whose snapshot and mass load together
 constexpr int kSubstepWaves = 6;  // waves per SIMD the substep kernel is compiled for
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
                     n_flat += hi - lo;
                     rend[r] = n_flat;
                 }
-                flat = n_flat <= 64u;
+                flat = n_flat <= kFlatMax;
             }
             auto flat_slot = [&](uint32_t i) {
                 uint32_t t = rlo[0] + i;
@@ -253,8 +254,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
                 return t < k.nb ? t : k.nb - 1u;
             };
             // candidates (flat walk or partner list) within the squared contact distance
-            uint64_t near = 0;
-            uint64_t lst = 0;  // flat candidates within the list reach (first substep after a rebuild)
+            uint64_t near = 0, near_hi = 0;  // candidates 0..63 and 64..127
+            uint64_t lst = 0, lst_hi = 0;    // flat candidates within the list reach (first substep after a rebuild)
             if (flat) {
                 for (uint32_t f = 0; f < n_flat; f += 4) {
                     uint32_t tt[4];
@@ -271,9 +272,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
                         const float d2 = dx * dx + dy * dy;
                         const bool base = f + u < n_flat && tt[u] != s && !(__float_as_uint(sb[u].w) & kSnapOutlier);
                         const bool cand = base && !(d2 > (rs * rs) * 1.00000095367431640625f);
-                        near |= cand ? (1ull << (f + u)) : 0ull;
                         const float rl = rs + list_reach;
-                        lst |= (base && d2 <= rl * rl) ? (1ull << (f + u)) : 0ull;
+                        const bool in_list = base && d2 <= rl * rl;
+                        const uint32_t b = f + (uint32_t)u;
+                        const uint64_t bit = 1ull << (b & 63u);
+                        if (b < 64u) {
+                            near |= cand ? bit : 0ull;
+                            lst |= in_list ? bit : 0ull;
+                        } else {
+                            near_hi |= cand ? bit : 0ull;
+                            lst_hi |= in_list ? bit : 0ull;
+                        }
                     }
                 }
             } else if (!GATHER) {
@@ -300,12 +309,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
             }
             if (GATHER) {
                 // this body's partner list for the substeps up to the next rebuild
-                uint32_t n_l = flat ? (uint32_t)__popcll(lst) : (uint32_t)kNbrNone;
+                uint32_t n_l = flat ? (uint32_t)(__popcll(lst) + __popcll(lst_hi)) : (uint32_t)kNbrNone;
                 if (n_l > kNbrCap) n_l = kNbrNone;
                 if (n_l != kNbrNone) {
                     uint32_t q = 0, word = 0;
-                    for (uint64_t m = lst; m != 0; m &= m - 1) {
-                        const int32_t dt = (int32_t)flat_slot((uint32_t)__builtin_ctzll(m)) - (int32_t)s;
+                    uint64_t m = lst, m2 = lst_hi;
+                    while ((m | m2) != 0) {
+                        const uint32_t i = m ? (uint32_t)__builtin_ctzll(m) : 64u + (uint32_t)__builtin_ctzll(m2);
+                        if (m) {
+                            m &= m - 1;
+                        } else {
+                            m2 &= m2 - 1;
+                        }
+                        const int32_t dt = (int32_t)flat_slot(i) - (int32_t)s;
                         if (dt < -32768 || dt > 32767) {
                             n_l = kNbrNone;
                             break;
@@ -325,15 +341,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
             auto enumerate = [&](auto&& sink) {
                 if (flat || use_list) {
                     // marked candidates four at a time: their snapshots and masses load together
-                    uint64_t m = near;
-                    while (m != 0) {
+                    uint64_t m = near, m2 = near_hi;
+                    while ((m | m2) != 0) {
                         uint32_t tt[kHitBatch];
                         bool ok[kHitBatch];
 #pragma unroll
                         for (int u = 0; u < kHitBatch; ++u) {
-                            ok[u] = m != 0;
-                            const uint32_t i = ok[u] ? (uint32_t)__builtin_ctzll(m) : 0u;
-                            m &= m - 1;
+                            ok[u] = (m | m2) != 0;
+                            const uint32_t i = m ? (uint32_t)__builtin_ctzll(m) : (m2 ? 64u + (uint32_t)__builtin_ctzll(m2) : 0u);
+                            if (m) {
+                                m &= m - 1;
+                            } else {
+                                m2 &= m2 - 1;
+                            }
                             tt[u] = ok[u] ? (GATHER ? flat_slot(i) : list_slot(i)) : s;
                         }
                         float4 sb[kHitBatch];

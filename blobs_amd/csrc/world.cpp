@@ -117,15 +117,16 @@ int World::debug_accum(uint32_t* out4) {
     return BG_OK;
 }
 
-// Histogram of the partner-list lengths now on the device: [0..16] bodies
-// with that many entries, [17] bodies without a list (they walk the grid).
-int World::debug_list_lengths(uint32_t* out18) {
-    for (int i = 0; i < 18; ++i) out18[i] = 0;
+// Histogram of the partner-list lengths now on the device: [0..kNbrCap]
+// bodies with that many entries, [kNbrCap + 1] bodies without a list (they
+// walk the grid).
+int World::debug_list_lengths(uint32_t* out) {
+    for (uint32_t i = 0; i < kNbrCap + 2; ++i) out[i] = 0;
     if (!d_.nbr_len || d_.nb == 0) return BG_OK;
     std::vector<uint8_t> len(d_.nb);
     HIP_TRY(hipMemcpyAsync(len.data(), d_.nbr_len, d_.nb, hipMemcpyDeviceToHost, d_.stream));
     HIP_TRY(hipStreamSynchronize(d_.stream));
-    for (uint8_t v : len) ++out18[v <= kNbrCap ? v : 17];
+    for (uint8_t v : len) ++out[v <= kNbrCap ? v : kNbrCap + 1];
     return BG_OK;
 }
 

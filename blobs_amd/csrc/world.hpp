@@ -103,7 +103,7 @@ public:
                    uint32_t* counts);
     int set_profiling(bool enable);
     int debug_accum(uint32_t* out4);
-    int debug_list_lengths(uint32_t* out18);
+    int debug_list_lengths(uint32_t* out);
     int stage_stats(uint32_t stage, uint64_t* launches, double* total_ms);
     uint32_t rbd_count() const { return bodies_.len(); }
     StepPath path() const { return path_; }
