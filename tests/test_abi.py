@@ -294,3 +294,20 @@ def test_rigid_body_force_and_impulse_helpers():
     s.apply_impulse_at_point((1.0, 1.0), (5.0, 5.0))
     s.apply_force_at_point((1.0, 1.0), (5.0, 5.0))
     assert s.acceleration == (0.0, 0.0) and s.velocity_request is None and s.torque == 0.0
+
+
+def test_c_program_links_and_drives_the_abi(tmp_path):
+    """tests/c_abi/abi_example.c, compiled against include/blobs_gpu.h and
+    linked with libblobs_gpu.so: construction, a reference panic as a status,
+    a stale handle, then a step (refused with BG_ERR_NO_DEVICE on a machine
+    without a GPU; run and checked on one)."""
+    import subprocess
+    exe = tmp_path / "abi_example"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-I", str(ROOT / "include"),
+                    str(ROOT / "tests" / "c_abi" / "abi_example.c"), "-L", str(ROOT / "blobs_amd"), "-lblobs_gpu",
+                    "-lm", f"-Wl,-rpath,{ROOT / 'blobs_amd'}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    if gpu_available():
+        assert r.returncode == 0, r.stdout + r.stderr
+    else:
+        assert r.returncode == 3 and "no HIP device" in r.stdout, r.stdout + r.stderr
