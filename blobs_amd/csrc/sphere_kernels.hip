@@ -209,8 +209,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
             const float Qc = ra + a.rmax + (GATHER ? list_reach : a.skin);
             const bool use_list = !GATHER && nlen <= kNbrCap && !was_out;
             // Sorted-grid candidates (first substep after a rebuild).  With at most kFlatRows
-            // rows and 64 candidates (the common case) they form one flat list: a cheap pass
-            // marks the candidates within the squared contact distance in a 64-bit mask, and
+            // rows and kFlatMax candidates (the common case) they form one flat list: a cheap
+            // pass marks the candidates within the squared contact distance in a bit mask, and
             // only marked ones take the exact test and the correction -- so a wave pays the
             // correction path once per hit of its busiest lane, not once per candidate.
             int32_t qcx0 = 0, qcx1 = 0, qcy0 = 0, qcy1 = 0;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
             }
             auto enumerate = [&](auto&& sink) {
                 if (flat || use_list) {
-                    // marked candidates four at a time: their snapshots and masses load together
+                    // marked candidates kHitBatch at a time: their snapshots and masses load together
                     uint64_t m = near, m2 = near_hi;
                     while ((m | m2) != 0) {
                         uint32_t tt[kHitBatch];
