@@ -150,6 +150,8 @@ struct SubstepArgs {
     uint32_t has_springs;
     uint32_t sub;       // sphere path: substeps since the last rebuild (1-based)
     uint32_t sub_global;  // substeps since the world was created (orders collision events)
+    uint32_t write_vel;   // sphere path: store calculated_velocity (last substep of a step, or
+                          // every substep when collision events read it)
 };
 
 // Launchers (step_kernels.hip: general path + broadphase build).

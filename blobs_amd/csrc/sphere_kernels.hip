@@ -552,7 +552,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubst
             if (fl_out != fl) k.dst.bflags[s] = fl_out;
             if (a.fresh && !(fl & BF_STATIC)) k.dst.torque[s] = 0.0f;
         }
-        k.vel_out[s] = vel;
+        if (a.write_vel) k.vel_out[s] = vel;
         // collider transform from the pre-constraint position (physics.rs:360-366)
         const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform_slow(st.rot, st.p, k.src.off[g_own]);
         const bool finite_x = finitef(x.x) && finitef(x.y);

@@ -1293,7 +1293,7 @@ int World::rebuild() {
     return BG_OK;
 }
 
-int World::run_substep(float h, bool fresh) {
+int World::run_substep(float h, bool fresh, bool last) {
     DeviceWorld& d = d_;
     SubstepArgs a;
     a.h = h;
@@ -1322,6 +1322,9 @@ int World::run_substep(float h, bool fresh) {
     a.skin = skin_;
     a.sub = ++sub_since_rebuild_;
     a.sub_global = substep_seq_++;
+    // calculated_velocity is read only after a step (getters, the next rebuild's
+    // statistics) or by collision events, which need every substep's
+    a.write_vel = (last || d.events) ? 1u : 0u;
     a.t_prev = (float)((double)(a.sub - 1) * (double)h);
     a.t_cur = (float)((double)a.sub * (double)h);
     if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
@@ -1375,7 +1378,7 @@ int World::launch_steps(double delta, uint32_t n) {
                 int st = rebuild();
                 if (st) return st;
             }
-            int st = run_substep(h, fresh_);
+            int st = run_substep(h, fresh_, sub + 1 == params_.substeps);
             if (st) return st;
             fresh_ = false;
             stepped_since_upload_ = true;

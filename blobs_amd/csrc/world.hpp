@@ -122,7 +122,7 @@ private:
     int sync_host() { return host_stale_ ? download() : BG_OK; }
     void mark_dirty() { dev_dirty_ = true; }
     int rebuild();
-    int run_substep(float h, bool fresh);
+    int run_substep(float h, bool fresh, bool last);
     int prepare_step();
     int launch_steps(double delta, uint32_t n);
     void advance_time(double delta, uint32_t n);
