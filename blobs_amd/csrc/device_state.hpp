@@ -120,6 +120,7 @@ struct DeviceWorld {
     GridParams *grid = nullptr;
     StepAccum *accum = nullptr;
     unsigned long long *block_contacts = nullptr;  // contact count per block of the per-body kernels
+                                                   // (ceil(nb / 64) slots: blocks of 64 or more threads)
     uint32_t n_block_slots = 0;
     SortTemp sort{};
 

@@ -42,6 +42,9 @@ constexpr uint32_t kFlatMax = 128;  // candidates of a flat list (two 64-bit mas
 constexpr int kContacts = 4;      // contacts summed from registers; more take a selection pass in key order
 constexpr int kHitBatch = 2;      // marked candidates whose snapshot and mass load together
 constexpr int kSubstepWaves = 6;  // waves per SIMD the substep kernel is compiled for
+constexpr int kSubstepThreads = 64;  // threads per block of the substep kernel (one wave)
+static_assert(kSubstepThreads >= 64 && kSubstepThreads % 64 == 0,
+              "block_contacts holds one slot per 64 bodies: substep blocks are whole waves");
 
 struct SphereSubstep {
     SphereArrays src;  // body records read (through perm on the first substep after a rebuild)
@@ -130,15 +133,15 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
 }
 
 template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER, bool BATCHED>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSubstepWaves))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
+__global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu(kSubstepWaves))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = blk * kThreads + threadIdx.x;
+    const uint32_t s = blk * kSubstepThreads + threadIdx.x;
     // clear the outlier table of the substep after next
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i <= k.oh_mask; i += gridDim.x * kThreads) {
+    for (uint32_t i = blockIdx.x * kSubstepThreads + threadIdx.x; i <= k.oh_mask; i += gridDim.x * kSubstepThreads) {
         k.oh_clr_head[i] = kNil;
         if ((i & 31u) == 0u) k.oh_clr_bits[i >> 5] = 0u;
     }
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < k.n_waves; i += gridDim.x * kThreads)
+    for (uint32_t i = blockIdx.x * kSubstepThreads + threadIdx.x; i < k.n_waves; i += gridDim.x * kSubstepThreads)
         k.oh_clr_wave[i] = 0u;
     // world of this slot (lanes past the end take the last body's world)
     const uint32_t w = BATCHED ? k.wid[s < k.nb ? s : k.nb - 1] : 0u;
@@ -945,8 +948,8 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.vel_out = k.events ? d.vel_next : k.dst.vel;
     k.nbr = d.nbr;
     k.nbr_len = d.nbr_len;
-    const uint32_t blocks = div_up(d.nb, kThreads);
-#define BG_SPHERE(G, S, Z, C, P, B) k_sphere_substep<G, S, Z, C, P, B><<<blocks, kThreads, 0, d.stream>>>(k, a)
+    const uint32_t blocks = div_up(d.nb, kSubstepThreads);
+#define BG_SPHERE(G, S, Z, C, P, B) k_sphere_substep<G, S, Z, C, P, B><<<blocks, kSubstepThreads, 0, d.stream>>>(k, a)
 #define BG_SPHERE_PB(G, S, Z, C)                 \
     do {                                         \
         if (gather && batched)                   \

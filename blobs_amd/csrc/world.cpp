@@ -1128,7 +1128,8 @@ int World::upload() {
         HIP_TRY(dmalloc(&d.events, event_cap_));
         if (sphere) HIP_TRY(dmalloc(&d.vel_next, nb));
     }
-    d.n_block_slots = (std::max<uint32_t>(nb, 1) + 255) / 256;
+    // one contact slot per block of any per-body launch (the smallest blocks have 64 threads)
+    d.n_block_slots = (std::max<uint32_t>(nb, 1) + 63) / 64;
     HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
     HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
     HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, cell_rows + 1)));
