@@ -658,15 +658,32 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
     const int lane = threadIdx.x & (BG_WAVE - 1), wv = threadIdx.x / BG_WAVE;
     const uint32_t w0 = __shfl(w, 0, BG_WAVE);
     const bool wave_uniform = __all(w == w0);
-    float* dst = wstat + ((size_t)w * rep + (blockIdx.x & (rep - 1))) * 8;
-    if (!wave_uniform) {  // a wave across worlds: per-lane atomics
-        if (v[7] > 0.0f) {
-            for (int t = 0; t < 8; ++t) atomicAdd(&dst[t], v[t]);
-            uint32_t* bb = wbbox + ((size_t)w * rep + (blockIdx.x & (rep - 1))) * 4;
-            atomic_min_if_less(&bb[0], b0);
-            atomic_min_if_less(&bb[1], b1);
-            atomic_max_if_greater(&bb[2], b2);
-            atomic_max_if_greater(&bb[3], b3);
+    if (!wave_uniform) {
+        // a wave across worlds (small worlds): one wave reduction per world in it, and
+        // one lane issues that world's update
+        uint64_t left = __ballot(1);
+        while (left != 0) {
+            const int l0 = __builtin_ctzll(left);
+            const uint32_t wc = __shfl(w, l0, BG_WAVE);
+            const bool mine = w == wc;
+            float r[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                r[t] = mine ? v[t] : 0.0f;
+                for (int o = 32; o > 0; o >>= 1) r[t] += __shfl_xor(r[t], o, BG_WAVE);
+            }
+            const uint32_t c0 = wave_min_u32(mine ? b0 : 0xFFFFFFFFu), c1 = wave_min_u32(mine ? b1 : 0xFFFFFFFFu);
+            const uint32_t c2 = wave_max_u32(mine ? b2 : 0u), c3 = wave_max_u32(mine ? b3 : 0u);
+            if (lane == l0 && r[7] > 0.0f) {
+                float* dc = wstat + ((size_t)wc * rep + (blockIdx.x & (rep - 1))) * 8;
+                for (int t = 0; t < 8; ++t) atomicAdd(&dc[t], r[t]);
+                uint32_t* bb = wbbox + ((size_t)wc * rep + (blockIdx.x & (rep - 1))) * 4;
+                atomic_min_if_less(&bb[0], c0);
+                atomic_min_if_less(&bb[1], c1);
+                atomic_max_if_greater(&bb[2], c2);
+                atomic_max_if_greater(&bb[3], c3);
+            }
+            left &= ~__ballot(mine);
         }
     } else {
 #pragma unroll
