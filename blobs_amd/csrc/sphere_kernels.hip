@@ -799,15 +799,35 @@ __global__ __launch_bounds__(kThreads) void k_sphere_count(const float4* __restr
                                                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
                                                            uint32_t* __restrict__ rank) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    const uint32_t w = BATCHED ? wid[i] : 0u;
-    const GridParams g = grids[w];
-    const float4 sn = snap[i];
-    const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
-    const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
-    const uint32_t kk = cell_base[w] + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
-    key[i] = kk;
-    rank[i] = atomicAdd(&cnt[kk], 1u);
+    const bool live = i < nb;
+    uint32_t kk = kNil;
+    if (live) {
+        const uint32_t w = BATCHED ? wid[i] : 0u;
+        const GridParams g = grids[w];
+        const float4 sn = snap[i];
+        const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
+        const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
+        kk = cell_base[w] + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+        key[i] = kk;
+    }
+    // Slots are still in the previous rebuild's cell order and bodies move little, so
+    // neighbouring lanes often share a cell: each run of equal keys takes its ranks with
+    // one atomic (the run's first lane), the others offset from it.  Order inside a cell
+    // is arbitrary.
+    const uint32_t lane = threadIdx.x & (BG_WAVE - 1);
+    const uint32_t prev = __shfl_up(kk, 1, BG_WAVE);
+    const bool head = live && (lane == 0u || prev != kk);
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);
+    const uint32_t my_head = 63u - (uint32_t)__builtin_clzll((heads & le) | 1ull);  // this lane's run start
+    const uint64_t above = heads & ~le;  // heads after this lane
+    const uint64_t live_mask = __ballot(live);
+    const uint32_t n_live = live_mask ? 64u - (uint32_t)__builtin_clzll(live_mask) : 0u;  // live lanes are a prefix
+    const uint32_t run_end = above ? (uint32_t)__builtin_ctzll(above) : n_live;
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cnt[kk], run_end - lane);
+    base = __shfl(base, (int)my_head, BG_WAVE);
+    if (live) rank[i] = base + (lane - my_head);
 }
 
 struct SphereScatter {
