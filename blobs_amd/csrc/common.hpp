@@ -154,6 +154,9 @@ enum : uint32_t {
     BF_INERTIA_POS = 1u << 2,  // inertia is positive and finite
     BF_SENSOR = 1u << 3,       // sphere path: the body's collider is a sensor
     BF_FIRST_DYN = 1u << 4,    // sphere path: first non-static body of its world (Q1, physics.rs:338-340)
+    BF_ROT_STILL = 1u << 5,    // sphere path: a substep without user torque leaves rotation and angular
+                               // velocity bit-unchanged (static, or omega +0, finite rotation that is
+                               // not -0, positive inertia: physics.rs:345-349 then adds +0)
 };
 
 struct GridParams {

@@ -158,8 +158,10 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
         const uint32_t id = wa & ~kSnapOutlier;
         const bool was_out = (wa & kSnapOutlier) != 0u;
         const float2 mgv = k.src.mg[g_own];
-        // loaded now, used after the contact pass: their latency overlaps the candidate walk
-        const float2 rw = k.src.rw[g_own];
+        // loaded now, used after the contact pass: their latency overlaps the candidate walk.
+        // A body whose rotation cannot change in this substep skips it.
+        const bool still = ZERO_OFF && !GATHER && !a.fresh && (fl & BF_ROT_STILL) != 0u;
+        const float2 rw = still ? make_float2(0.0f, 0.0f) : k.src.rw[g_own];
         const float2 x0 = k.x0[s];
         const uint32_t nlen = GATHER ? (uint32_t)kNbrNone : (uint32_t)k.nbr_len[s];
         // an outlier of the previous substep may touch a body of this wave
@@ -539,6 +541,12 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             vel = st.vel;
             if (vr) fl_out = fl & ~BF_HAS_VREQ;
         }
+        if (ZERO_OFF && !still) {
+            const bool now_still = (fl & BF_STATIC) != 0u ||
+                                   (__float_as_uint(st.omega) == 0u && __float_as_uint(st.rot) != 0x80000000u &&
+                                    finitef(st.rot) && (fl & BF_INERTIA_POS) != 0u);
+            fl_out = now_still ? (fl_out | BF_ROT_STILL) : (fl_out & ~BF_ROT_STILL);
+        }
         if (GATHER) {
             // first substep after a rebuild: every record moves to its new slot
             k.dst.rw[s] = make_float2(st.rot, st.omega);
@@ -550,14 +558,17 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             if (GROUPS) k.dst.groups[s] = k.src.groups[g_own];
             if (fl_out & BF_HAS_VREQ) k.dst.vreq[s] = k.src.vreq[g_own];
         } else {
-            if (__float_as_uint(st.rot) != __float_as_uint(rw.x) || __float_as_uint(st.omega) != __float_as_uint(rw.y))
+            if (!still && (__float_as_uint(st.rot) != __float_as_uint(rw.x) ||
+                           __float_as_uint(st.omega) != __float_as_uint(rw.y)))
                 k.dst.rw[s] = make_float2(st.rot, st.omega);
             if (fl_out != fl) k.dst.bflags[s] = fl_out;
             if (a.fresh && !(fl & BF_STATIC)) k.dst.torque[s] = 0.0f;
         }
         if (a.write_vel) k.vel_out[s] = vel;
         // collider transform from the pre-constraint position (physics.rs:360-366)
-        const float2 x = ZERO_OFF ? body_xform_zero_off(st.rot, st.p) : body_xform_slow(st.rot, st.p, k.src.off[g_own]);
+        // (a still body's rotation is needed only for a position with a zero coordinate)
+        const float rot_x = (still && !(st.p.x != 0.0f && st.p.y != 0.0f)) ? k.src.rw[g_own].x : st.rot;
+        const float2 x = ZERO_OFF ? body_xform_zero_off(rot_x, st.p) : body_xform_slow(st.rot, st.p, k.src.off[g_own]);
         const bool finite_x = finitef(x.x) && finitef(x.y);
         bool out = was_out;
         if (a.collisions && !out && finite_x) {

@@ -915,8 +915,15 @@ int World::upload() {
         if (bc_start[i + 1] - bc_start[i] != 1 || bc_list[bc_start[i]] != i || cpar[i] != (int32_t)i) sphere = false;
     path_ = sphere ? StepPath::Sphere : StepPath::General;
     if (sphere) {
-        for (uint32_t i = 0; i < nb; ++i)
+        for (uint32_t i = 0; i < nb; ++i) {
             if (cfl[i] & 1u) bflags[i] |= BF_SENSOR;
+            uint32_t ob, rb;
+            memcpy(&ob, &omega[i], 4);
+            memcpy(&rb, &rot[i], 4);
+            if ((bflags[i] & BF_STATIC) ||
+                (ob == 0u && rb != 0x80000000u && std::isfinite(rot[i]) && (bflags[i] & BF_INERTIA_POS)))
+                bflags[i] |= BF_ROT_STILL;
+        }
     }
     // common drift for the broadphase query centre: worth it when every
     // moving body feels the same gravity
