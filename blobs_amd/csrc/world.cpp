@@ -1021,6 +1021,9 @@ int World::upload() {
             HIP_TRY(dmalloc(&q.vreq, nb));
         }
         HIP_TRY(dmalloc(&d.snap_next, nb));
+        // the kernels index the entry-major lists with 32-bit offsets
+        if ((uint64_t)(kNbrCap / 2) * nb > 0xFFFFFFFFull)
+            return fail(BG_ERR_CAPACITY, "too many bodies for one fused sphere world (partner-list offsets are 32-bit)");
         HIP_TRY(dmalloc(&d.nbr, (size_t)(kNbrCap / 2) * nb));
         HIP_TRY(dmalloc(&d.nbr_len, nb));
         HIP_TRY(hipMemsetAsync(d.nbr_len, kNbrNone, nb, s));
