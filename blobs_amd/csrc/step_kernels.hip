@@ -28,12 +28,14 @@ namespace bg {
 namespace {
 
 // Cell key of every collider snapshot on the grid fitted to the bounding box
-// accumulated since the previous rebuild; x0 keeps the snapshot.
+// accumulated since the previous rebuild, and its rank among its cell's
+// colliders (counting sort; order inside a cell is arbitrary, no result
+// depends on it); x0 keeps the snapshot.
 __global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float2* __restrict__ xy, uint32_t nc,
                                                            StepAccum* accum, uint32_t parity, float cell,
                                                            uint32_t cells_cap, GridParams* grid_out,
-                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                           float2* __restrict__ x0) {
+                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
+                                                           uint32_t* cnt, float2* __restrict__ x0) {
     const GridParams g = grid_from_bbox(accum->bbox[parity], cell, cells_cap);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -49,30 +51,26 @@ __global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float2* __restr
         const float2 x = xy[i];
         int32_t cx = cell_coord(x.x, g.ox, g.inv_cell, g.nx);
         int32_t cy = cell_coord(x.y, g.oy, g.inv_cell, g.ny);
-        keys[i] = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
-        vals[i] = i;
+        const uint32_t kk = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+        keys[i] = kk;
+        rank[i] = atomicAdd(&cnt[kk], 1u);
         x0[i] = x;
     }
 }
 
-// cell_start[c] = first sorted slot with key >= c, for c in [0, ncells]:
-// one thread per cell, lower_bound over the sorted keys (balanced however
-// the bodies are spread over the grid).
-__global__ __launch_bounds__(kThreads) void k_cell_start(const uint32_t* __restrict__ keys, uint32_t n,
-                                                         const GridParams* __restrict__ grid,
-                                                         uint32_t* __restrict__ cell_start) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t ncells = grid->ncells;
-    if (c > ncells) return;
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (keys[mid] < c)
-            lo = mid + 1;
-        else
-            hi = mid;
+// Colliders into cell order: slot = cell_start[key] + rank (cell_start is the
+// exclusive scan of the counts); the counts are cleared for the next rebuild.
+__global__ __launch_bounds__(kThreads) void k_cell_scatter(const uint32_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ rank, uint32_t nc,
+                                                           const uint32_t* __restrict__ cell_start,
+                                                           uint32_t* __restrict__ vals, uint32_t* cnt,
+                                                           uint32_t n_cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t c = i; c < n_cnt; c += gridDim.x * blockDim.x) cnt[c] = 0u;
+    if (i < nc) {
+        const uint32_t slot = cell_start[keys[i]] + rank[i];
+        if (slot < nc) vals[slot] = i;  // consistent counts and scan never fail this
     }
-    cell_start[c] = lo;
 }
 
 // ============================================================ general path
@@ -391,14 +389,18 @@ hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity) {
     uint32_t n = d.nc;
     uint32_t blocks = div_up(n > 0 ? n : 1, kThreads);
     k_rebuild_keys<<<blocks, kThreads, 0, d.stream>>>(d.c.abs, n, d.accum, parity, cell, d.cells_cap, d.grid, d.keys,
-                                                      d.vals, d.x0);
+                                                      d.vals_tmp, d.cell_count, d.x0);
     return hipGetLastError();
 }
 
+// Counting sort by cell: cell_start = exclusive scan of the counts (every
+// entry up to cells_cap, so cell_start[ncells] = nc), then the scatter.
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d) {
-    hipError_t e = radix_sort_pairs(&d.keys, &d.vals, &d.keys_tmp, &d.vals_tmp, d.nc, d.key_bits, d.sort, d.stream);
+    hipError_t e = exclusive_scan_u32(d.cell_count, d.cell_start, d.cells_cap + 1, d.sort, d.stream);
     if (e != hipSuccess) return e;
-    k_cell_start<<<div_up(d.cells_cap + 1, kThreads), kThreads, 0, d.stream>>>(d.keys, d.nc, d.grid, d.cell_start);
+    const uint32_t n = d.nc > 0 ? d.nc : 1u;
+    k_cell_scatter<<<div_up(std::max(n, d.cells_cap + 1), kThreads), kThreads, 0, d.stream>>>(
+        d.keys, d.vals_tmp, d.nc, d.cell_start, d.vals, d.cell_count, d.cells_cap + 1);
     return hipGetLastError();
 }
 

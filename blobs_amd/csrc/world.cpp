@@ -1131,6 +1131,10 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.keys_tmp, nc));
     HIP_TRY(dmalloc(&d.vals_tmp, nc));
     HIP_TRY(dmalloc(&d.cell_start, (size_t)cell_rows + 1));
+    if (!sphere) {  // the general path's cell counts (counting sort by cell)
+        HIP_TRY(dmalloc(&d.cell_count, (size_t)d.cells_cap + 1));
+        HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), s));
+    }
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
     if (event_cap_ > 0) {
