@@ -144,7 +144,7 @@ __device__ __forceinline__ void atomic_min_if_less(uint32_t* p, uint32_t v) {
 }
 
 // Contacts go to a per-block slot (summed on the host when queried), never
-// to one shared counter.
+// to one shared counter.  Blocks of 64 to kThreads threads.
 static __device__ void flush_accum(const LocalAccum& l, StepAccum* g, unsigned long long* block_contacts, uint32_t parity,
                             bool with_points) {
     __shared__ uint32_t s_red[8][kThreads / BG_WAVE];
@@ -172,7 +172,7 @@ static __device__ void flush_accum(const LocalAccum& l, StepAccum* g, unsigned l
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kThreads / BG_WAVE; ++w) {
+        for (int w = 1; w < (int)(blockDim.x / BG_WAVE); ++w) {
             s_red[0][0] = s_red[0][w] > s_red[0][0] ? s_red[0][w] : s_red[0][0];
             s_red[1][0] = s_red[1][w] < s_red[1][0] ? s_red[1][w] : s_red[1][0];
             s_red[2][0] = s_red[2][w] < s_red[2][0] ? s_red[2][w] : s_red[2][0];

@@ -27,6 +27,9 @@ namespace bg {
 
 namespace {
 
+constexpr uint32_t kContactThreads = 64;  // block size of k_contacts (block_contacts: one slot per 64 bodies)
+
+
 // Cell key of every collider snapshot on the grid fitted to the bounding box
 // accumulated since the previous rebuild, and its rank among its cell's
 // colliders (counting sort; order inside a cell is arbitrary, no result
@@ -410,14 +413,17 @@ hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t
     const uint32_t blocks = div_up(d.nb, kThreads);
     if (a.has_springs) k_forces<<<blocks, kThreads, 0, d.stream>>>(d, a);
     if (a.collisions && d.nc > 0) {
+        // one-wave blocks: a body's work varies with its collider count, so small blocks
+        // keep the slow waves from holding whole workgroups
+        const uint32_t cb = div_up(d.nb, kContactThreads);
         if (groups && sensors)
-            k_contacts<true, true><<<blocks, kThreads, 0, d.stream>>>(d, a);
+            k_contacts<true, true><<<cb, kContactThreads, 0, d.stream>>>(d, a);
         else if (groups)
-            k_contacts<true, false><<<blocks, kThreads, 0, d.stream>>>(d, a);
+            k_contacts<true, false><<<cb, kContactThreads, 0, d.stream>>>(d, a);
         else if (sensors)
-            k_contacts<false, true><<<blocks, kThreads, 0, d.stream>>>(d, a);
+            k_contacts<false, true><<<cb, kContactThreads, 0, d.stream>>>(d, a);
         else
-            k_contacts<false, false><<<blocks, kThreads, 0, d.stream>>>(d, a);
+            k_contacts<false, false><<<cb, kContactThreads, 0, d.stream>>>(d, a);
     }
     if (d.n_jcomp > 0 && joint_iterations > 0)
         k_joints<<<div_up(d.n_jcomp, 64), 64, 0, d.stream>>>(d, a, joint_iterations);
