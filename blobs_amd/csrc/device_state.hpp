@@ -53,6 +53,8 @@ constexpr uint32_t kNil = 0xFFFFFFFFu;
 constexpr uint32_t kNbrCap = 32;      // partner-list entries per body (sphere path)
 constexpr uint8_t kNbrNone = 0xFF;    // no list: the body walks the grid
 
+constexpr uint32_t kJointBodies = 16, kJointJoints = 16;
+
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
     uint32_t cells_cap = 0;
@@ -111,12 +113,18 @@ struct DeviceWorld {
     float *j_dist = nullptr, *j_target = nullptr;
     uint32_t *jc_start = nullptr, *jc_list = nullptr;
     uint32_t n_jcomp = 0;
+    // components of at most kJointBodies bodies and kJointJoints joints iterate in LDS:
+    // their bodies (jb_start/jb_list, empty for a larger component) and, per entry of
+    // jc_list, the joint's two bodies as local indices (a | b << 8)
+    uint32_t *jb_start = nullptr, *j_local = nullptr;
+    int32_t *jb_list = nullptr;
     // constraints
     float2 *con_pos = nullptr;
     float *con_r = nullptr;
     // broadphase
     uint32_t *keys = nullptr, *vals = nullptr, *keys_tmp = nullptr, *vals_tmp = nullptr;
     uint32_t *cell_start = nullptr;
+    float4 *crec = nullptr;  // general path: (x, y, radius, parent bits) of each collider, in cell order
     GridParams *grid = nullptr;
     StepAccum *accum = nullptr;
     unsigned long long *block_contacts = nullptr;  // contact count per block of the per-body kernels

@@ -63,16 +63,25 @@ __global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float2* __restr
 
 // Colliders into cell order: slot = cell_start[key] + rank (cell_start is the
 // exclusive scan of the counts); the counts are cleared for the next rebuild.
+// Beside the collider index, each slot gets the record the contact pass tests
+// (position, radius, parent body), so a candidate is one load from a cell run.
 __global__ __launch_bounds__(kThreads) void k_cell_scatter(const uint32_t* __restrict__ keys,
                                                            const uint32_t* __restrict__ rank, uint32_t nc,
                                                            const uint32_t* __restrict__ cell_start,
                                                            uint32_t* __restrict__ vals, uint32_t* cnt,
-                                                           uint32_t n_cnt) {
+                                                           uint32_t n_cnt, const float2* __restrict__ abs,
+                                                           const float* __restrict__ radius,
+                                                           const int32_t* __restrict__ parent,
+                                                           float4* __restrict__ rec) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t c = i; c < n_cnt; c += gridDim.x * blockDim.x) cnt[c] = 0u;
     if (i < nc) {
         const uint32_t slot = cell_start[keys[i]] + rank[i];
-        if (slot < nc) vals[slot] = i;  // consistent counts and scan never fail this
+        if (slot < nc) {  // consistent counts and scan never fail this
+            vals[slot] = i;
+            const float2 x = abs[i];
+            rec[slot] = make_float4(x.x, x.y, radius[i], __int_as_float(parent[i]));
+        }
     }
 }
 
@@ -140,23 +149,27 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
                 const bool sen_a = SENSORS ? (d.c.cflags[c] & 1u) : false;
                 const float Q = ra + a.rmax + D;
                 scan_stencil(g, d.cell_start, d.nc, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, Q, [&](uint32_t t) {
-                    const uint32_t j = d.vals[t];
-                    if (j == c || j >= d.nc) return;
-                    const int32_t pj = d.c.parent[j];
+                    // slot t's record (the sort ran on this substep's positions); the collider
+                    // itself and its body's other colliders share the parent b
+                    const float4 rb = d.crec[t];
+                    const int32_t pj = __float_as_int(rb.w);
                     if (pj < 0 || pj == (int32_t)b) return;
+                    // axis = A - B with A the higher collider index; b - a == -(a - b) up to
+                    // the sign of a zero, so the unoriented difference decides and the axis
+                    // is formed for a hit only
+                    const float dx = xa.x - rb.x, dy = xa.y - rb.y;
+                    const float rs = ra + rb.z;
+                    const float dot = dx * dx + dy * dy;
+                    if (dot > (rs * rs) * 1.00000095367431640625f) return;
+                    const float dist = sqrtf(dot);
+                    if (!(dist < rs)) return;
+                    const uint32_t j = d.vals[t];
                     if (GROUPS) {
                         uint2 gb = d.c.groups[j];
                         if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) return;
                     }
-                    const float2 xb = d.c.abs[j];
                     const bool isA = c > j;
-                    float ax = isA ? xa.x - xb.x : xb.x - xa.x;
-                    float ay = isA ? xa.y - xb.y : xb.y - xa.y;
-                    const float rs = ra + d.c.radius[j];
-                    float dot = ax * ax + ay * ay;
-                    if (dot > (rs * rs) * 1.00000095367431640625f) return;
-                    float dist = sqrtf(dot);
-                    if (!(dist < rs)) return;
+                    const float ax = isA ? dx : rb.x - xa.x, ay = isA ? dy : rb.y - xa.y;
                     const uint64_t key = isA ? (((uint64_t)c << 32) | j) : (((uint64_t)j << 32) | c);
                     emit(key, isA, j, pj, ax, ay, dist, rs, sen_a);
                 });
@@ -270,57 +283,123 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
     flush_accum(la, d.accum, d.block_contacts, 0, false);
 }
 
+// One fixed joint (physics.rs:424-477) on body positions pa, pb and rotations
+// ra, rb (read before any write of this joint).  Returns false for coincident
+// anchors, which the reference skips; the caller then writes nothing.
+__device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, float& rb, float2 an_a, float2 an_b,
+                                            float jdist, float jtarget, float ma, float mb, bool sa, bool sb, float h,
+                                            uint32_t& flags) {
+    float wax = pa.x + an_a.x, way = pa.y + an_a.y;
+    float wbx = pb.x + an_b.x, wby = pb.y + an_b.y;
+    float dx = wbx - wax, dy = wby - way;
+    float dist = vlen(dx, dy);
+    if (dist < 1e-6f) return false;
+    float off_by = dist - jdist;
+    float cx = (off_by * dx) / dist, cy = (off_by * dy) / dist;
+    if (!(ma > 0.0f) || !(mb > 0.0f)) flags |= ERR_JOINT_MASS;
+    float inv_a = 1.0f / ma, inv_b = 1.0f / mb;
+    float ims = inv_a + inv_b;
+    if (sa) {
+        pb.x = pb.x - ims * cx;
+        pb.y = pb.y - ims * cy;
+    } else if (sb) {
+        pa.x = pa.x + ims * cx;
+        pa.y = pa.y + ims * cy;
+    } else {
+        float ratio = inv_a / ims;
+        float rbm = 1.0f - ratio;
+        pa.x = pa.x + ratio * cx;
+        pa.y = pa.y + ratio * cy;
+        pb.x = pb.x - rbm * cx;
+        pb.y = pb.y - rbm * cy;
+    }
+    float angle_a = atan2_f32(dy, dx);
+    float angle_b = -atan2_f32(dy, -dx);
+    float diff = (angle_b - angle_a) - jtarget;
+    float corr = diff * 0.5f;
+    ra = ra + corr * h;
+    rb = rb - corr * h;
+    if (!finitef(ra) || !finitef(rb)) flags |= ERR_JOINT_ROT;
+    return true;
+}
+
 // Fixed joints (physics.rs:424-477): Gauss-Seidel in arena order inside each
 // component of the joint graph; components touch disjoint bodies, so running
-// them concurrently reproduces the sequential sweep.
+// them concurrently reproduces the sequential sweep.  One thread per
+// component.  A component of at most kJointBodies bodies and kJointJoints
+// joints iterates on copies of its bodies and joints in LDS (the sweep is a
+// chain of dependent reads and writes, so each step costs an LDS round trip
+// instead of a cache round trip) and writes its bodies back once; a larger
+// one iterates in global memory.  Writes keep the reference's order (a's
+// position, b's, a's rotation, b's), which decides a joint of a body to itself.
 __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uint32_t iterations) {
-    uint32_t comp = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float2 s_pos[kJointBodies][64];
+    __shared__ float s_rot[kJointBodies][64], s_mass[kJointBodies][64];
+    __shared__ float4 s_anchor[kJointJoints][64];  // anchor a, anchor b
+    __shared__ float2 s_len[kJointJoints][64];     // distance, target angle
+    __shared__ uint32_t s_loc[kJointJoints][64];   // local a | local b << 8
+    const uint32_t lane = threadIdx.x;
+    const uint32_t comp = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t flags = 0;
     if (comp < d.n_jcomp) {
         const uint32_t k0 = d.jc_start[comp], k1 = d.jc_start[comp + 1];
-        for (uint32_t it = 0; it < iterations; ++it) {
-            for (uint32_t kk = k0; kk < k1; ++kk) {
-                const uint32_t j = d.jc_list[kk];
-                const int32_t ia = d.j_a[j], ib = d.j_b[j];
-                float2 pa = d.b.pos[ia], pb = d.b.pos[ib];
+        const uint32_t b0 = d.jb_start[comp], nbl = d.jb_start[comp + 1] - b0;
+        if (nbl > 0) {
+            uint32_t stat = 0;  // bit l: local body l is static
+            for (uint32_t l = 0; l < nbl; ++l) {
+                const int32_t g = d.jb_list[b0 + l];
+                s_pos[l][lane] = d.b.pos[g];
+                s_rot[l][lane] = d.b.rot[g];
+                s_mass[l][lane] = d.b.mass[g];
+                if (d.b.bflags[g] & BF_STATIC) stat |= 1u << l;
+            }
+            const uint32_t nq = k1 - k0;
+            for (uint32_t q = 0; q < nq; ++q) {
+                const uint32_t j = d.jc_list[k0 + q];
                 const float2 an_a = d.j_anchor_a[j], an_b = d.j_anchor_b[j];
-                float wax = pa.x + an_a.x, way = pa.y + an_a.y;
-                float wbx = pb.x + an_b.x, wby = pb.y + an_b.y;
-                float dx = wbx - wax, dy = wby - way;
-                float dist = vlen(dx, dy);
-                if (dist < 1e-6f) continue;
-                float off_by = dist - d.j_dist[j];
-                float cx = (off_by * dx) / dist, cy = (off_by * dy) / dist;
-                float ma = d.b.mass[ia], mb = d.b.mass[ib];
-                if (!(ma > 0.0f) || !(mb > 0.0f)) flags |= ERR_JOINT_MASS;
-                float inv_a = 1.0f / ma, inv_b = 1.0f / mb;
-                float ims = inv_a + inv_b;
-                bool sa = d.b.bflags[ia] & BF_STATIC, sb = d.b.bflags[ib] & BF_STATIC;
-                if (sa) {
-                    pb.x = pb.x - ims * cx;
-                    pb.y = pb.y - ims * cy;
-                } else if (sb) {
-                    pa.x = pa.x + ims * cx;
-                    pa.y = pa.y + ims * cy;
-                } else {
-                    float ratio = inv_a / ims;
-                    float rb = 1.0f - ratio;
-                    pa.x = pa.x + ratio * cx;
-                    pa.y = pa.y + ratio * cy;
-                    pb.x = pb.x - rb * cx;
-                    pb.y = pb.y - rb * cy;
+                s_anchor[q][lane] = make_float4(an_a.x, an_a.y, an_b.x, an_b.y);
+                s_len[q][lane] = make_float2(d.j_dist[j], d.j_target[j]);
+                s_loc[q][lane] = d.j_local[k0 + q];
+            }
+            for (uint32_t it = 0; it < iterations; ++it) {
+                for (uint32_t q = 0; q < nq; ++q) {
+                    const uint32_t loc = s_loc[q][lane];
+                    const uint32_t la = loc & 0xFFu, lb = (loc >> 8) & 0xFFu;
+                    const float4 an = s_anchor[q][lane];
+                    const float2 len = s_len[q][lane];
+                    float2 pa = s_pos[la][lane], pb = s_pos[lb][lane];
+                    float ra = s_rot[la][lane], rb = s_rot[lb][lane];
+                    if (!joint_solve(pa, pb, ra, rb, make_float2(an.x, an.y), make_float2(an.z, an.w), len.x, len.y,
+                                     s_mass[la][lane], s_mass[lb][lane], (stat >> la) & 1u, (stat >> lb) & 1u, a.h,
+                                     flags))
+                        continue;
+                    s_pos[la][lane] = pa;
+                    s_pos[lb][lane] = pb;
+                    s_rot[la][lane] = ra;
+                    s_rot[lb][lane] = rb;
                 }
-                d.b.pos[ia] = pa;
-                d.b.pos[ib] = pb;
-                float angle_a = atan2_f32(dy, dx);
-                float angle_b = -atan2_f32(dy, -dx);
-                float diff = (angle_b - angle_a) - d.j_target[j];
-                float corr = diff * 0.5f;
-                float ra = d.b.rot[ia] + corr * a.h;
-                float rb2 = d.b.rot[ib] - corr * a.h;
-                d.b.rot[ia] = ra;
-                d.b.rot[ib] = rb2;
-                if (!finitef(ra) || !finitef(rb2)) flags |= ERR_JOINT_ROT;
+            }
+            for (uint32_t l = 0; l < nbl; ++l) {
+                const int32_t g = d.jb_list[b0 + l];
+                d.b.pos[g] = s_pos[l][lane];
+                d.b.rot[g] = s_rot[l][lane];
+            }
+        } else {
+            for (uint32_t it = 0; it < iterations; ++it) {
+                for (uint32_t kk = k0; kk < k1; ++kk) {
+                    const uint32_t j = d.jc_list[kk];
+                    const int32_t ia = d.j_a[j], ib = d.j_b[j];
+                    float2 pa = d.b.pos[ia], pb = d.b.pos[ib];
+                    float ra = d.b.rot[ia], rb = d.b.rot[ib];
+                    if (!joint_solve(pa, pb, ra, rb, d.j_anchor_a[j], d.j_anchor_b[j], d.j_dist[j], d.j_target[j],
+                                     d.b.mass[ia], d.b.mass[ib], d.b.bflags[ia] & BF_STATIC,
+                                     d.b.bflags[ib] & BF_STATIC, a.h, flags))
+                        continue;
+                    d.b.pos[ia] = pa;
+                    d.b.pos[ib] = pb;
+                    d.b.rot[ia] = ra;
+                    d.b.rot[ib] = rb;
+                }
             }
         }
     }
@@ -403,7 +482,8 @@ hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d) {
     if (e != hipSuccess) return e;
     const uint32_t n = d.nc > 0 ? d.nc : 1u;
     k_cell_scatter<<<div_up(std::max(n, d.cells_cap + 1), kThreads), kThreads, 0, d.stream>>>(
-        d.keys, d.vals_tmp, d.nc, d.cell_start, d.vals, d.cell_count, d.cells_cap + 1);
+        d.keys, d.vals_tmp, d.nc, d.cell_start, d.vals, d.cell_count, d.cells_cap + 1, d.c.abs, d.c.radius,
+        d.c.parent, d.crec);
     return hipGetLastError();
 }
 

@@ -675,8 +675,9 @@ void World::free_arrays() {
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
-    dfree(d.jc_start), dfree(d.jc_list), dfree(d.con_pos), dfree(d.con_r);
-    dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start);
+    dfree(d.jc_start), dfree(d.jc_list), dfree(d.jb_start), dfree(d.jb_list), dfree(d.j_local), dfree(d.con_pos),
+        dfree(d.con_r);
+    dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.crec);
     dfree(d.grid), dfree(d.accum), dfree(d.block_contacts), dfree(d.events), dfree(d.vel_next);
     sort_temp_free(d.sort);
     hipStream_t s = d.stream;
@@ -865,7 +866,8 @@ int World::upload() {
         j_dist.push_back(j.distance);
         j_target.push_back(j.target_angle);
     }
-    std::vector<uint32_t> jc_start, jc_list;
+    std::vector<uint32_t> jc_start, jc_list, jb_start, j_local;
+    std::vector<int32_t> jb_list;
     uint32_t n_jcomp = 0;
     if (nj > 0 && !structure_error_) {
         std::vector<int32_t> uf(nb);
@@ -898,6 +900,28 @@ int World::upload() {
         jc_list.resize(nj);
         std::vector<uint32_t> fillp(jc_start.begin(), jc_start.end() - 1);
         for (uint32_t k = 0; k < nj; ++k) jc_list[fillp[jcomp[k]]++] = k;
+        // local body indices of the components small enough to iterate in LDS
+        jb_start.assign(n_jcomp + 1, 0);
+        j_local.assign(nj, 0);
+        std::vector<int32_t> local_of(nb, -1), bodies;
+        for (uint32_t c = 0; c < n_jcomp; ++c) {
+            bodies.clear();
+            for (uint32_t kk = jc_start[c]; kk < jc_start[c + 1]; ++kk) {
+                const uint32_t j = jc_list[kk];
+                for (int32_t bi : {j_a[j], j_b[j]})
+                    if (local_of[bi] < 0) {
+                        local_of[bi] = (int32_t)bodies.size();
+                        bodies.push_back(bi);
+                    }
+                j_local[kk] = (uint32_t)(local_of[j_a[j]] & 0xFF) | ((uint32_t)(local_of[j_b[j]] & 0xFF) << 8);
+            }
+            const bool fits = jc_start[c + 1] - jc_start[c] <= kJointJoints && bodies.size() <= kJointBodies;
+            for (int32_t bi : bodies) {
+                local_of[bi] = -1;
+                if (fits) jb_list.push_back(bi);
+            }
+            jb_start[c + 1] = (uint32_t)jb_list.size();
+        }
     }
 
     // constraints
@@ -1124,6 +1148,9 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.j_target, nj));
     HIP_TRY(dmalloc(&d.jc_start, n_jcomp + 1));
     HIP_TRY(dmalloc(&d.jc_list, nj));
+    HIP_TRY(dmalloc(&d.jb_start, n_jcomp + 1));
+    HIP_TRY(dmalloc(&d.jb_list, jb_list.size()));
+    HIP_TRY(dmalloc(&d.j_local, nj));
     HIP_TRY(dmalloc(&d.con_pos, nk));
     HIP_TRY(dmalloc(&d.con_r, nk));
     HIP_TRY(dmalloc(&d.keys, nc));
@@ -1134,6 +1161,7 @@ int World::upload() {
     if (!sphere) {  // the general path's cell counts (counting sort by cell)
         HIP_TRY(dmalloc(&d.cell_count, (size_t)d.cells_cap + 1));
         HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), s));
+        HIP_TRY(dmalloc(&d.crec, nc));
     }
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
@@ -1167,6 +1195,9 @@ int World::upload() {
     HIP_TRY(h2d(d.j_target, j_target, s));
     HIP_TRY(h2d(d.jc_start, jc_start, s));
     HIP_TRY(h2d(d.jc_list, jc_list, s));
+    HIP_TRY(h2d(d.jb_start, jb_start, s));
+    HIP_TRY(h2d(d.jb_list, jb_list, s));
+    HIP_TRY(h2d(d.j_local, j_local, s));
     HIP_TRY(h2d(d.con_pos, con_pos, s));
     HIP_TRY(h2d(d.con_r, con_r, s));
     StepAccum acc0;

@@ -251,3 +251,62 @@ def test_empty_and_single_body_worlds():
     h = p.insert_rbd(RigidBodyBuilder().position((0.0, 0.0)).build())
     p.step(1 / 60)
     assert p.get_rbd(h).position[1] < 0.0
+
+
+def test_joint_components_small_and_large():
+    """Joint components on both sides of the size at which the device keeps a
+    component in LDS (16 bodies and 16 joints): chains of 1-40 joints, stars
+    around a static hub with 15 and 16 spokes, a ring and a repeated pair.
+    Colliders have no offset, so rotations (whose atan2 may differ from
+    glibc's in the last bit) never reach positions: positions are bit-exact
+    over several steps; rotations agree to 1e-5."""
+    rng = np.random.default_rng(11)
+    pos, ja, jb, static = [], [], [], []
+
+    def body(x, y, is_static=False):
+        pos.append((x, y))
+        static.append(is_static)
+        return len(pos) - 1
+
+    row = 0.0
+    for length in (1, 2, 5, 15, 16, 40):  # 15 joints: 16 bodies (LDS); 16: 17 bodies (global)
+        ids = [body(1.0 * k, row) for k in range(length + 1)]
+        ja += ids[:-1]
+        jb += ids[1:]
+        row += 3.0
+    for spokes in (15, 16):
+        hub = body(0.0, row, True)
+        for k in range(spokes):
+            ang = 2 * np.pi * k / spokes
+            ja.append(hub)
+            jb.append(body(2.5 * np.cos(ang), row + 2.5 * np.sin(ang)))
+        row += 8.0
+    ring = [body(2.0 * np.cos(2 * np.pi * k / 8), row + 2.0 * np.sin(2 * np.pi * k / 8)) for k in range(8)]
+    ja += ring
+    jb += ring[1:] + ring[:1]
+    row += 6.0
+    a, b = body(0.0, row), body(1.2, row)
+    ja += [a, b]
+    jb += [b, a]
+    pos = np.array(pos, np.float32)
+    n = pos.shape[0]
+    vel = rng.normal(0, 2, (n, 2)).astype(np.float32)
+    rad = np.full(n, 0.4, np.float32)
+    sc = scenes._single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1 / 60, 3, {}, static_mask=np.array(static))
+    sc.joint_a = np.array(ja, np.int32)
+    sc.joint_b = np.array(jb, np.int32)
+    nj = sc.joint_a.shape[0]
+    sc.joint_anchor_a = rng.uniform(-0.2, 0.2, (nj, 2)).astype(np.float32)
+    sc.joint_anchor_b = rng.uniform(-0.2, 0.2, (nj, 2)).astype(np.float32)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    for step in range(3):
+        p.step(sc.dt)
+        o.step(sc.dt)
+        assert p.step_path() == "general"
+        b = p.bodies()
+        assert_bitexact(b["position"], o.pos, f"step {step} position")
+        rot_err = np.abs(b["rotation"].astype(np.float64) - o.rot.astype(np.float64))
+        assert rot_err.max() < 1e-5, rot_err.max()
+    moved = np.abs(p.bodies()["position"] - pos).max(axis=1)
+    assert (moved[~np.array(static)] > 0).all()
