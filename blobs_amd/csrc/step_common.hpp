@@ -74,11 +74,13 @@ struct ContactList {
 };
 
 // Visit every slot in the cells overlapping the square of half-width Q
-// around (qx, qy).  Cell coordinates are monotone in x, so the visited set
-// is a superset of all snapshots within distance Q of the query point.
+// around (qx, qy), in slot order, with the slot's record.  Cell coordinates
+// are monotone in x, so the visited set is a superset of all snapshots within
+// distance Q of the query point.  Records are loaded four at a time so that
+// their loads overlap.
 template <class F>
 __device__ __forceinline__ void scan_stencil(const GridParams& g, const uint32_t* __restrict__ cell_start, uint32_t n,
-                                             float qx, float qy, float Q, F&& visit) {
+                                             const float4* __restrict__ rec, float qx, float qy, float Q, F&& visit) {
     float R = Q + (1e-5f * (fabsf(qx) + fabsf(qy) + Q) + 1e-6f);
     int32_t cx0, cx1, cy0, cy1;
     if (R < 1e30f && finitef(qx) && finitef(qy)) {
@@ -97,7 +99,14 @@ __device__ __forceinline__ void scan_stencil(const GridParams& g, const uint32_t
         uint32_t t0 = cell_start[row + (uint32_t)cx0];
         uint32_t t1 = cell_start[row + (uint32_t)cx1 + 1u];
         t1 = t1 < n ? t1 : n;  // cell_start is a monotone table in [0, n]
-        for (uint32_t t = t0; t < t1; ++t) visit(t);
+        for (uint32_t t = t0; t < t1; t += 4) {
+            float4 r[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) r[u] = rec[min(t + u, t1 - 1u)];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u)
+                if (t + u < t1) visit(t + u, r[u]);
+        }
     }
 }
 

@@ -148,10 +148,10 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
                 if (GROUPS) ga = d.c.groups[c];
                 const bool sen_a = SENSORS ? (d.c.cflags[c] & 1u) : false;
                 const float Q = ra + a.rmax + D;
-                scan_stencil(g, d.cell_start, d.nc, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, Q, [&](uint32_t t) {
+                scan_stencil(g, d.cell_start, d.nc, d.crec, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, Q,
+                             [&](uint32_t t, const float4 rb) {
                     // slot t's record (the sort ran on this substep's positions); the collider
                     // itself and its body's other colliders share the parent b
-                    const float4 rb = d.crec[t];
                     const int32_t pj = __float_as_int(rb.w);
                     if (pj < 0 || pj == (int32_t)b) return;
                     // axis = A - B with A the higher collider index; b - a == -(a - b) up to
@@ -335,6 +335,7 @@ __device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, f
 __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uint32_t iterations) {
     __shared__ float2 s_pos[kJointBodies][64];
     __shared__ float s_rot[kJointBodies][64], s_mass[kJointBodies][64];
+    __shared__ int32_t s_gid[kJointBodies][64];    // body index
     __shared__ float4 s_anchor[kJointJoints][64];  // anchor a, anchor b
     __shared__ float2 s_len[kJointJoints][64];     // distance, target angle
     __shared__ uint32_t s_loc[kJointJoints][64];   // local a | local b << 8
@@ -345,21 +346,36 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
         const uint32_t k0 = d.jc_start[comp], k1 = d.jc_start[comp + 1];
         const uint32_t b0 = d.jb_start[comp], nbl = d.jb_start[comp + 1] - b0;
         if (nbl > 0) {
+            // copies in, eight entries per round so that their loads overlap (an index past
+            // the end repeats the last entry, rewriting its slot with the same values)
             uint32_t stat = 0;  // bit l: local body l is static
-            for (uint32_t l = 0; l < nbl; ++l) {
-                const int32_t g = d.jb_list[b0 + l];
-                s_pos[l][lane] = d.b.pos[g];
-                s_rot[l][lane] = d.b.rot[g];
-                s_mass[l][lane] = d.b.mass[g];
-                if (d.b.bflags[g] & BF_STATIC) stat |= 1u << l;
+            for (uint32_t l0 = 0; l0 < nbl; l0 += 8) {
+                int32_t g[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) g[u] = d.jb_list[b0 + min(l0 + u, nbl - 1)];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const uint32_t l = min(l0 + u, nbl - 1);
+                    s_gid[l][lane] = g[u];
+                    s_pos[l][lane] = d.b.pos[g[u]];
+                    s_rot[l][lane] = d.b.rot[g[u]];
+                    s_mass[l][lane] = d.b.mass[g[u]];
+                    if (d.b.bflags[g[u]] & BF_STATIC) stat |= 1u << l;
+                }
             }
             const uint32_t nq = k1 - k0;
-            for (uint32_t q = 0; q < nq; ++q) {
-                const uint32_t j = d.jc_list[k0 + q];
-                const float2 an_a = d.j_anchor_a[j], an_b = d.j_anchor_b[j];
-                s_anchor[q][lane] = make_float4(an_a.x, an_a.y, an_b.x, an_b.y);
-                s_len[q][lane] = make_float2(d.j_dist[j], d.j_target[j]);
-                s_loc[q][lane] = d.j_local[k0 + q];
+            for (uint32_t q0 = 0; q0 < nq; q0 += 8) {
+                uint32_t j[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) j[u] = d.jc_list[k0 + min(q0 + u, nq - 1)];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const uint32_t q = min(q0 + u, nq - 1);
+                    const float2 an_a = d.j_anchor_a[j[u]], an_b = d.j_anchor_b[j[u]];
+                    s_anchor[q][lane] = make_float4(an_a.x, an_a.y, an_b.x, an_b.y);
+                    s_len[q][lane] = make_float2(d.j_dist[j[u]], d.j_target[j[u]]);
+                    s_loc[q][lane] = d.j_local[k0 + q];
+                }
             }
             for (uint32_t it = 0; it < iterations; ++it) {
                 for (uint32_t q = 0; q < nq; ++q) {
@@ -380,7 +396,7 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
                 }
             }
             for (uint32_t l = 0; l < nbl; ++l) {
-                const int32_t g = d.jb_list[b0 + l];
+                const int32_t g = s_gid[l][lane];
                 d.b.pos[g] = s_pos[l][lane];
                 d.b.rot[g] = s_rot[l][lane];
             }
@@ -449,13 +465,26 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
             if (vr) d.b.bflags[b] = fl & ~BF_HAS_VREQ;
             if (a.fresh) d.b.torque[b] = 0.0f;
         }
-        // collider transforms from the pre-constraint position (:360-366)
-        for (uint32_t kk = d.bc_start[b]; kk < d.bc_start[b + 1]; ++kk) {
-            uint32_t c = d.bc_list[kk];
-            float2 x = body_xform(s.rot, s.p, d.c.off[c]);
-            d.c.abs[c] = x;
-            float2 x0 = d.x0[c];
-            la.point(x.x, x.y, x0.x, x0.y, a.drift_x, a.drift_y);
+        // collider transforms from the pre-constraint position (:360-366), four colliders per
+        // round so that their loads overlap (an index past the end repeats the last collider:
+        // the same store, and the bounding box and displacement maximum are idempotent)
+        const uint32_t c0 = d.bc_start[b], c1 = d.bc_start[b + 1];
+        for (uint32_t k0 = c0; k0 < c1; k0 += 4) {
+            uint32_t c[4];
+            float2 off[4], x0[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) c[u] = d.bc_list[min(k0 + u, c1 - 1u)];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                off[u] = d.c.off[c[u]];
+                x0[u] = d.x0[c[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const float2 x = body_xform(s.rot, s.p, off[u]);
+                d.c.abs[c[u]] = x;
+                la.point(x.x, x.y, x0[u].x, x0[u].y, a.drift_x, a.drift_y);
+            }
         }
         apply_constraints(s.p, d.con_pos, d.con_r, d.nk);
         d.b.pos[b] = s.p;
