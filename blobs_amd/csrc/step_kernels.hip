@@ -248,31 +248,26 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
             });
         }
         if (L.n > 0) {
+            const uint32_t total = L.n;
             uint64_t last = L.apply(p);
-            uint32_t done = L.n < 16 ? L.n : 16;
-            // more corrections than the register list holds: apply the rest
-            // by repeated selection of the next key (same order)
-            while (done < L.n) {
-                uint64_t best = ~0ull;
-                float bx = 0.0f, by = 0.0f;
+            uint32_t done = total < 16 ? total : 16;
+            // more corrections than the register list holds: each further pass over
+            // the candidates collects the 16 smallest keys above the last one applied
+            // and applies them, so the order stays ascending
+            while (done < total) {
+                L.init();
                 enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist,
                               float rs, bool sen_a) {
                     correction(
                         false,
                         [&](uint64_t k, float x, float y) {
-                            if (k > last && k < best) {
-                                best = k;
-                                bx = x;
-                                by = y;
-                            }
+                            if (k > last) L.add(k, x, y);
                         },
                         key, isA, j, pj, ax, ay, dist, rs, sen_a);
                 });
-                if (best == ~0ull) break;
-                p.x = p.x + bx;
-                p.y = p.y + by;
-                last = best;
-                ++done;
+                if (L.n == 0) break;
+                last = L.apply(p);
+                done += L.n < 16 ? L.n : 16;
             }
             d.b.pos[b] = p;
         }

@@ -310,3 +310,31 @@ def test_joint_components_small_and_large():
         assert rot_err.max() < 1e-5, rot_err.max()
     moved = np.abs(p.bodies()["position"] - pos).max(axis=1)
     assert (moved[~np.array(static)] > 0).all()
+
+
+def test_crowded_bodies_beyond_the_contact_lists_bitexact():
+    """160 spheres of radius 0.8-1.6 in a 6 x 6 box: every body overlaps most
+    of the others (about 75 contacts each), far more than either path's
+    register list holds.  The sphere path and the general path (forced by a
+    zero spring, which adds +0 to accelerations) apply the rest in further
+    passes over the candidates, in the same ascending order, so both agree
+    bit for bit with the oracle."""
+    rng = np.random.default_rng(7)
+    n = 160
+    pos = rng.uniform(0.0, 6.0, (n, 2)).astype(np.float32)
+    vel = rng.normal(0.0, 0.5, (n, 2)).astype(np.float32)
+    rad = rng.uniform(0.8, 1.6, n).astype(np.float32)
+    sc = scenes._single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1 / 60, 3, {})
+    o = OracleWorld(sc, np.float32)
+    p1 = gpu_world(sc)
+    p2, bh = sc.build_physics()
+    p2.springs.insert(Spring(int(bh[0]), int(bh[1]), 0.0, 0.0, 0.0))
+    for step in range(3):
+        o.step(sc.dt)
+        p1.step(sc.dt)
+        p2.step(sc.dt)
+        assert p1.step_path() == "sphere" and p2.step_path() == "general"
+        assert p1.contact_count() == p2.contact_count() == o.contact_count
+        assert_bitexact(p1.bodies()["position"], o.pos, f"step {step} sphere path")
+        assert_bitexact(p2.bodies()["position"], o.pos, f"step {step} general path")
+    assert o.contact_count > 25 * n  # pairs: over 50 contacts per body on average
