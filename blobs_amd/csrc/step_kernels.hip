@@ -123,11 +123,18 @@ __global__ __launch_bounds__(kThreads) void k_forces(DeviceWorld d, SubstepArgs 
     d.b.acc[b] = acc;
 }
 
-// Contacts (physics.rs:241-317), one thread per body over its colliders.
+constexpr uint32_t kColStage = 8;  // colliders per body staged in LDS by k_contacts
+
+// Contacts (physics.rs:241-317), one thread per body over its colliders, in
+// blocks of kContactThreads.
 template <bool GROUPS, bool SENSORS>
-__global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArgs a) {
+__global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, SubstepArgs a) {
+    // the body's first kColStage colliders (x, y, radius, index), loaded together up front
+    // so that the walk over its colliders does not wait on two dependent loads per collider
+    __shared__ float4 s_col[kColStage][kContactThreads];
     LocalAccum la;
     la.init();
+    const uint32_t lane = threadIdx.x;
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b < d.nb) {
         const GridParams g = *d.grid;
@@ -137,13 +144,34 @@ __global__ __launch_bounds__(kThreads) void k_contacts(DeviceWorld d, SubstepArg
         ContactList<16, uint64_t> L;
         L.init();
         const uint32_t k0 = d.bc_start[b], k1 = d.bc_start[b + 1];
+        if (k1 > k0) {
+            // an index past the end repeats the last collider (same slot, same values)
+            uint32_t cc[kColStage];
+#pragma unroll
+            for (uint32_t u = 0; u < kColStage; ++u) cc[u] = d.bc_list[min(k0 + u, k1 - 1u)];
+#pragma unroll
+            for (uint32_t u = 0; u < kColStage; ++u) {
+                const float2 x = d.c.abs[cc[u]];
+                s_col[min(u, k1 - k0 - 1u)][lane] = make_float4(x.x, x.y, d.c.radius[cc[u]], __uint_as_float(cc[u]));
+            }
+        }
         // one enumeration of this body's contact corrections, in visit order
         auto enumerate = [&](auto&& emit) {
             for (uint32_t kk = k0; kk < k1; ++kk) {
-                const uint32_t c = d.bc_list[kk];
-                const float2 xa = d.c.abs[c];
+                uint32_t c;
+                float2 xa;
+                float ra;
+                if (kk - k0 < kColStage) {
+                    const float4 sc = s_col[kk - k0][lane];
+                    c = __float_as_uint(sc.w);
+                    xa = make_float2(sc.x, sc.y);
+                    ra = sc.z;
+                } else {
+                    c = d.bc_list[kk];
+                    xa = d.c.abs[c];
+                    ra = d.c.radius[c];
+                }
                 if (!finitef(xa.x) || !finitef(xa.y)) continue;
-                const float ra = d.c.radius[c];
                 uint2 ga = make_uint2(0, 0);
                 if (GROUPS) ga = d.c.groups[c];
                 const bool sen_a = SENSORS ? (d.c.cflags[c] & 1u) : false;
