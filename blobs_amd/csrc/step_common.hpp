@@ -74,10 +74,11 @@ struct ContactList {
 };
 
 // Visit every slot in the cells overlapping the square of half-width Q
-// around (qx, qy), in slot order, with the slot's record.  Cell coordinates
-// are monotone in x, so the visited set is a superset of all snapshots within
-// distance Q of the query point.  Records are loaded four at a time so that
-// their loads overlap.
+// around (qx, qy), in slot order, four at a time: visit(t, r, nv) gets slots
+// t .. t+nv-1 (nv in 1..4) with their records r[0..nv-1] (loaded together,
+// so their loads overlap).  Cell coordinates are monotone in x, so the
+// visited set is a superset of all snapshots within distance Q of the query
+// point.
 template <class F>
 __device__ __forceinline__ void scan_stencil(const GridParams& g, const uint32_t* __restrict__ cell_start, uint32_t n,
                                              const float4* __restrict__ rec, float qx, float qy, float Q, F&& visit) {
@@ -103,9 +104,7 @@ __device__ __forceinline__ void scan_stencil(const GridParams& g, const uint32_t
             float4 r[4];
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) r[u] = rec[min(t + u, t1 - 1u)];
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u)
-                if (t + u < t1) visit(t + u, r[u]);
+            visit(t, r, t1 - t < 4u ? t1 - t : 4u);
         }
     }
 }

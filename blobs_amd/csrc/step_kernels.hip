@@ -177,29 +177,51 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
                 const bool sen_a = SENSORS ? (d.c.cflags[c] & 1u) : false;
                 const float Q = ra + a.rmax + D;
                 scan_stencil(g, d.cell_start, d.nc, d.crec, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, Q,
-                             [&](uint32_t t, const float4 rb) {
-                    // slot t's record (the sort ran on this substep's positions); the collider
-                    // itself and its body's other colliders share the parent b
-                    const int32_t pj = __float_as_int(rb.w);
-                    if (pj < 0 || pj == (int32_t)b) return;
+                             [&](uint32_t t, const float4 (&rb)[4], uint32_t nv) {
+                    // slot records (the sort ran on this substep's positions); the collider
+                    // itself and its body's other colliders share the parent b.  All four
+                    // are tested first, then the hits' collider indices and partner masses
+                    // are loaded together, then the hits are taken in slot order.
                     // axis = A - B with A the higher collider index; b - a == -(a - b) up to
                     // the sign of a zero, so the unoriented difference decides and the axis
                     // is formed for a hit only
-                    const float dx = xa.x - rb.x, dy = xa.y - rb.y;
-                    const float rs = ra + rb.z;
-                    const float dot = dx * dx + dy * dy;
-                    if (dot > (rs * rs) * 1.00000095367431640625f) return;
-                    const float dist = sqrtf(dot);
-                    if (!(dist < rs)) return;
-                    const uint32_t j = d.vals[t];
-                    if (GROUPS) {
-                        uint2 gb = d.c.groups[j];
-                        if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) return;
+                    bool hit[4];
+                    float dx[4], dy[4], dist[4], rs[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const int32_t pj = __float_as_int(rb[u].w);
+                        dx[u] = xa.x - rb[u].x;
+                        dy[u] = xa.y - rb[u].y;
+                        rs[u] = ra + rb[u].z;
+                        const float dot = dx[u] * dx[u] + dy[u] * dy[u];
+                        dist[u] = sqrtf(dot);
+                        hit[u] = u < nv && pj >= 0 && pj != (int32_t)b &&
+                                 !(dot > (rs[u] * rs[u]) * 1.00000095367431640625f) && dist[u] < rs[u];
                     }
-                    const bool isA = c > j;
-                    const float ax = isA ? dx : rb.x - xa.x, ay = isA ? dy : rb.y - xa.y;
-                    const uint64_t key = isA ? (((uint64_t)c << 32) | j) : (((uint64_t)j << 32) | c);
-                    emit(key, isA, j, pj, ax, ay, dist, rs, sen_a);
+                    uint32_t jj[4];
+                    float mp[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        jj[u] = 0u;
+                        mp[u] = 0.0f;
+                        if (hit[u]) {
+                            jj[u] = d.vals[t + u];
+                            mp[u] = d.b.mass[__float_as_int(rb[u].w)];
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        if (!hit[u]) continue;
+                        const uint32_t j = jj[u];
+                        if (GROUPS) {
+                            uint2 gb = d.c.groups[j];
+                            if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) continue;
+                        }
+                        const bool isA = c > j;
+                        const float ax = isA ? dx[u] : rb[u].x - xa.x, ay = isA ? dy[u] : rb[u].y - xa.y;
+                        const uint64_t key = isA ? (((uint64_t)c << 32) | j) : (((uint64_t)j << 32) | c);
+                        emit(key, isA, j, __float_as_int(rb[u].w), ax, ay, dist[u], rs[u], sen_a, mp[u]);
+                    }
                 });
             }
         };
@@ -207,7 +229,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
         // Correction vector of one contact for this body's side.  A push-out
         // (physics.rs:272-286) is folded into the same entry: it is order
         // dependent in the reference and reproduced here for the pair only.
-        auto correction = [&](bool first, auto&& sink, uint64_t key, bool isA, uint32_t j, int32_t pj, float ax,
+        auto correction = [&](bool first, auto&& sink, uint64_t key, bool isA, uint32_t j, float mpart, float ax,
                               float ay, float dist, float rs, bool sen_a) {
             float px = 0.0f, py = 0.0f;
             const bool pushed = dist < 1e-6f;
@@ -226,7 +248,6 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
                 if (pushed) sink(key, px, py);
                 return;
             }
-            const float mpart = d.b.mass[pj];
             const float mA = isA ? mown : mpart, mB = isA ? mpart : mown;
             const float ratio = 1.0f - mA / (mA + mB);
             const float nx = ax / dist, ny = ay / dist;
@@ -249,9 +270,9 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
             sink(key, cx, cy);
         };
         enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist, float rs,
-                      bool sen_a) {
+                      bool sen_a, float mp) {
             if (isA) ++count;
-            correction(true, [&](uint64_t k, float x, float y) { L.add(k, x, y); }, key, isA, j, pj, ax, ay, dist,
+            correction(true, [&](uint64_t k, float x, float y) { L.add(k, x, y); }, key, isA, j, mp, ax, ay, dist,
                        rs, sen_a);
         });
         if (d.events && count > 0) {
@@ -260,7 +281,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
             uint32_t e = atomicAdd(&d.accum->event_count, count);
             const float2 va = d.b.vel[b];
             enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist, float rs,
-                          bool sen_a) {
+                          bool sen_a, float) {
                 if (!isA) return;
                 if (e < d.event_cap) {
                     DevEvent ev;
@@ -285,13 +306,13 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
             while (done < total) {
                 L.init();
                 enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist,
-                              float rs, bool sen_a) {
+                              float rs, bool sen_a, float mp) {
                     correction(
                         false,
                         [&](uint64_t k, float x, float y) {
                             if (k > last) L.add(k, x, y);
                         },
-                        key, isA, j, pj, ax, ay, dist, rs, sen_a);
+                        key, isA, j, mp, ax, ay, dist, rs, sen_a);
                 });
                 if (L.n == 0) break;
                 last = L.apply(p);
