@@ -39,7 +39,7 @@ namespace {
 constexpr int kCoarseShift = 2;  // outlier table cells are 4x4 grid cells
 constexpr int kFlatRows = 6;      // grid rows a query walks as one flat candidate list
 constexpr uint32_t kFlatMax = 128;  // candidates of a flat list (two 64-bit masks)
-constexpr int kContacts = 4;      // contacts summed from registers; more take further passes, kContacts each
+constexpr int kContacts = 4;      // contacts summed from registers; more take a selection pass in key order
 constexpr int kHitBatch = 2;      // marked candidates whose snapshot and mass load together
 constexpr int kSubstepWaves = 6;  // waves per SIMD the substep kernel is compiled for
 constexpr int kSubstepThreads = 64;  // threads per block of the substep kernel (one wave)
@@ -486,25 +486,29 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                 });
             }
             if (L.n > 0) {
-                const uint32_t total = L.n;
                 uint32_t last = L.apply(st.p);
-                uint32_t done = total < (uint32_t)kContacts ? total : (uint32_t)kContacts;
-                // more than kContacts: each further pass over the hits collects the
-                // kContacts smallest keys above the last one applied, same order
-                while (done < total) {
-                    L.init();
+                uint32_t done = L.n < (uint32_t)kContacts ? L.n : (uint32_t)kContacts;
+                while (done < L.n) {  // more than kContacts: next key by selection, same order
+                    uint32_t best = kNil;
+                    float bx = 0.0f, by = 0.0f;
                     enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs,
                                   float mp) {
                         correction(
                             false,
                             [&](uint32_t kk, float x, float y) {
-                                if (kk > last) L.add(kk, x, y);
+                                if (kk > last && kk < best) {
+                                    best = kk;
+                                    bx = x;
+                                    by = y;
+                                }
                             },
                             t, idb, isA, ax, ay, dist, rs, mp);
                     });
-                    if (L.n == 0) break;
-                    last = L.apply(st.p);
-                    done += L.n < (uint32_t)kContacts ? L.n : (uint32_t)kContacts;
+                    if (best == kNil) break;
+                    st.p.x = st.p.x + bx;
+                    st.p.y = st.p.y + by;
+                    last = best;
+                    ++done;
                 }
             }
         } else if (GATHER) {
