@@ -338,3 +338,37 @@ def test_crowded_bodies_beyond_the_contact_lists_bitexact():
         assert_bitexact(p1.bodies()["position"], o.pos, f"step {step} sphere path")
         assert_bitexact(p2.bodies()["position"], o.pos, f"step {step} general path")
     assert o.contact_count > 25 * n  # pairs: over 50 contacts per body on average
+
+
+def test_bodies_with_many_colliders_bitexact():
+    """Bodies of 12 colliders each (the general contact pass stages a body's
+    first 8 colliders and reads the rest from memory).  Without joints or
+    torques rotations stay exactly 0, so collider transforms are exact and
+    the general path is bit-exact against the oracle."""
+    rng = np.random.default_rng(12)
+    n, per = 400, 12
+    bpos = rng.uniform(0.0, 30.0, (n, 2)).astype(np.float32)
+    bvel = rng.normal(0.0, 1.0, (n, 2)).astype(np.float32)
+    ang = 2 * np.pi * np.arange(per) / per
+    ring = np.stack([0.8 * np.cos(ang), 0.8 * np.sin(ang)], 1).astype(np.float32)
+    parent = np.repeat(np.arange(n, dtype=np.int32), per)
+    off = np.tile(ring, (n, 1))
+    nc = n * per
+    cr = rng.uniform(0.2, 0.4, nc).astype(np.float32)
+    sc = scenes.Scene(
+        pos=bpos, vreq=bvel, has_vreq=np.ones(n, np.uint8), gravity_mod=np.ones(n, np.float32),
+        body_type=np.zeros(n, np.uint8), rotation=np.zeros(n, np.float32), col_parent=parent, col_off=off,
+        col_abs=(bpos[parent] + off).astype(np.float32), col_r=cr, col_mass_override=np.zeros(nc, np.float32),
+        col_has_mo=np.zeros(nc, np.uint8), col_sensor=np.zeros(nc, np.uint8),
+        col_mem=np.full(nc, 0xFFFFFFFF, np.uint32), col_filt=np.full(nc, 0xFFFFFFFF, np.uint32),
+        gravity=(0.0, -9.81), dt=1.0 / 60.0, steps=3, meta={})
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    for step in range(3):
+        p.step(sc.dt)
+        o.step(sc.dt)
+        assert p.step_path() == "general"
+        assert p.contact_count() == o.contact_count
+        assert_bitexact(p.bodies()["position"], o.pos, f"step {step} position")
+    assert o.contact_count > n
+    assert (p.bodies()["rotation"] == 0.0).all()
