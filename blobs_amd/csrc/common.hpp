@@ -124,12 +124,12 @@ struct StepAccum {
     uint32_t pad0;
     uint32_t bbox[2][4];  // [parity]{minx, miny, maxx, maxy} as f2ord
     uint32_t n_outliers;  // sphere path: bodies beyond the skin since the last rebuild
-    uint32_t event_count;  // collision events recorded since the last drain (may exceed the buffer)
     uint32_t error_flags;
+    unsigned long long event_count;  // collision events recorded since the last drain (may exceed the buffer)
     uint32_t push_outs;
     float old_dt;  // Physics::old_dt (physics.rs:33), world scalar
-    uint32_t pad1;
 };
+static_assert(sizeof(StepAccum) == 64, "StepAccum layout");
 
 // One collision event (physics.rs:302-311): the substep it happened in, the
 // dense indices of collider A (the higher index) and B, and the bodies'
@@ -144,7 +144,9 @@ enum : uint32_t {
     ERR_JOINT_MASS = 1u << 1,   // physics.rs:447-448
     ERR_JOINT_ROT = 1u << 2,    // physics.rs:471-474
     ERR_CONTACT_CAP = 1u << 3,
-    INFO_PUSH_OUT = 1u << 8,    // physics.rs:272-286 (order-dependent, handled pairwise)
+    ERR_ORDERED_CAP = 1u << 4,  // a push-out cluster outgrew the ordered contact pass's buffers
+    INFO_PUSH_OUT = 1u << 8,    // physics.rs:272-286: a pair's snapshots coincided
+    INFO_PUSH_OUT_PAIRWISE = 1u << 9,  // ... on the fused sphere path, resolved pair by pair
 };
 
 // Body flag bits (device SoA `bflags`).

@@ -55,6 +55,40 @@ constexpr uint8_t kNbrNone = 0xFF;    // no list: the body walks the grid
 
 constexpr uint32_t kJointBodies = 16, kJointJoints = 16;
 
+// Ordered contact pass (general path).  A push-out (physics.rs:272-286)
+// rewrites the two colliders' snapshots from their bodies' live positions,
+// and every later pair of the (i, j<i) loop reads the rewritten values, so
+// the pass stops being order-independent once a pair's snapshots coincide.
+// k_contacts evaluates every pair on the substep's snapshot and records the
+// rows (A colliders) of coincident pairs; k_contacts_ordered then replays
+// the pairs whose outcome can differ ("dirty" pairs: a coincident pair, or a
+// pair one of whose colliders was rewritten at an earlier key) in reference
+// order and re-sums the positions of the bodies they touch.
+struct OrderedPair {
+    unsigned long long key;  // (A << 32) | B, A the higher collider index
+    int32_t pa, pb;          // parent bodies of A and B
+    uint32_t flags;          // OP_* bits
+    uint32_t pad;
+    float2 ca, cb;           // contact corrections of pa and pb (the push-out is separate)
+};
+enum : uint32_t {
+    OP_CONTACT = 1u,  // a contact when replayed in order
+    OP_PUSHED = 2u,   // took the push-out branch
+    OP_ORIG = 4u,     // a contact on the substep's snapshot (counted and sent by k_contacts)
+    OP_SENSOR = 8u,   // a sensor is involved: no correction
+};
+// One addend of a body's ordered position sum.
+struct OrderedTerm {
+    unsigned long long key;
+    float2 push;  // push-out, added first (OT_PUSH)
+    float2 c;     // contact correction (OT_CORR)
+    uint32_t flags;
+    uint32_t pad;
+};
+enum : uint32_t { OT_PUSH = 1u, OT_CORR = 2u };
+constexpr uint32_t kOrdCand = 8192;   // candidate colliders of one replayed row
+constexpr uint32_t kOrdTerms = 8192;  // addends of one body's position sum
+
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
     uint32_t cells_cap = 0;
@@ -126,6 +160,21 @@ struct DeviceWorld {
     uint32_t *cell_start = nullptr;
     float4 *crec = nullptr;  // general path: (x, y, radius, parent bits) of each collider, in cell order
     GridParams *grid = nullptr;
+    // ordered contact pass (general path; see OrderedPair)
+    uint32_t *ord_rows = nullptr;            // [nc] rows of coincident pairs, count in ord_n[0]
+    uint32_t *ord_n = nullptr;               // [1]
+    uint32_t *ord_mark = nullptr;            // [ceil(nc / 32)] row queued (bitmap)
+    float2 *pos_pre = nullptr;               // [nb] body positions before the contact pass
+    unsigned long long *mv_key = nullptr;    // [nc] key of a collider's first snapshot rewrite (~0: none)
+    float2 *mv_abs = nullptr;                // [nc] its current rewritten snapshot
+    uint32_t *mv_list = nullptr;             // [nc] rewritten colliders
+    uint32_t *ord_heap = nullptr;            // [nc] rows to replay (min-heap)
+    OrderedPair *ord_log = nullptr;          // dirty pairs in key order
+    uint32_t ord_log_cap = 0;
+    uint32_t *aff_list = nullptr;            // [nb] bodies of dirty pairs
+    uint32_t *aff_mark = nullptr;            // [ceil(nb / 32)]
+    uint32_t *ord_cand = nullptr;            // [kOrdCand]
+    OrderedTerm *ord_terms = nullptr;        // [kOrdTerms]
     StepAccum *accum = nullptr;
     unsigned long long *block_contacts = nullptr;  // contact count per block of the per-body kernels
                                                    // (ceil(nb / 64) slots: blocks of 64 or more threads)
@@ -172,6 +221,9 @@ hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t
 hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity);
 hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool groups, bool sensors, bool zero_off,
                                  bool constraints);
+// Launcher (ordered_contacts.hip): the ordered replay after k_contacts (exits at once
+// when no pair of the substep's snapshot coincides).
+hipError_t launch_contacts_ordered(DeviceWorld& d, const SubstepArgs& a, bool groups, bool sensors);
 // fills n copies of v (graph-safe replacement of hipMemsetAsync)
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 

@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             if (k.events && count > 0) {
                 // collision events of the pairs this body is A in, with both bodies'
                 // velocities before the contact pass (physics.rs:288-311)
-                uint32_t e = atomicAdd(&k.accum->event_count, count);
+                unsigned long long e = atomicAdd(&k.accum->event_count, (unsigned long long)count);
                 const float2 va = k.src.vel[g_own];
                 enumerate([&](uint32_t t, uint32_t idb, bool isA, float ax, float ay, float dist, float rs, float) {
                     if (!isA) return;

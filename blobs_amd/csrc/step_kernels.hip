@@ -140,6 +140,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
         const GridParams g = *d.grid;
         const float D = read_dmax(d.accum);
         float2 p = d.b.pos[b];
+        d.pos_pre[b] = p;  // the ordered pass re-sums from here if a push-out happens
         const float mown = d.b.mass[b];
         ContactList<16, uint64_t> L;
         L.init();
@@ -225,10 +226,13 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
                 });
             }
         };
-        uint32_t count = 0, pushes = 0, flags = 0;
-        // Correction vector of one contact for this body's side.  A push-out
-        // (physics.rs:272-286) is folded into the same entry: it is order
-        // dependent in the reference and reproduced here for the pair only.
+        uint32_t count = 0, flags = 0;
+        // Correction vector of one contact for this body's side.  A pair whose
+        // snapshots coincide takes the push-out branch (physics.rs:272-286), which
+        // makes the rest of the pass order dependent: its row (the A collider) is
+        // recorded once for k_contacts_ordered, which replays the affected pairs in
+        // reference order and rewrites the positions they touch.  The value summed
+        // here for such a pair is a placeholder.
         auto correction = [&](bool first, auto&& sink, uint64_t key, bool isA, uint32_t j, float mpart, float ax,
                               float ay, float dist, float rs, bool sen_a) {
             float px = 0.0f, py = 0.0f;
@@ -239,8 +243,11 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
                 px = isA ? 0.01f : -0.01f;
                 py = isA ? 0.0f : -0.0f;
                 if (first) {
-                    ++pushes;
                     flags |= INFO_PUSH_OUT;
+                    if (isA) {
+                        const uint32_t row = (uint32_t)(key >> 32), bit = 1u << (row & 31u);
+                        if (!(atomicOr(&d.ord_mark[row >> 5], bit) & bit)) d.ord_rows[atomicAdd(d.ord_n, 1u)] = row;
+                    }
                 }
             }
             const bool sen_b = SENSORS ? (d.c.cflags[j] & 1u) : false;
@@ -278,7 +285,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
         if (d.events && count > 0) {
             // collision events of the pairs whose A collider is this body's (physics.rs:288-311);
             // velocities are the bodies' calculated_velocity before the contact pass
-            uint32_t e = atomicAdd(&d.accum->event_count, count);
+            unsigned long long e = atomicAdd(&d.accum->event_count, (unsigned long long)count);
             const float2 va = d.b.vel[b];
             enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist, float rs,
                           bool sen_a, float) {
@@ -321,7 +328,6 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
             d.b.pos[b] = p;
         }
         la.contacts = count;
-        la.push_outs = pushes;
         la.flags = flags;
     }
     flush_accum(la, d.accum, d.block_contacts, 0, false);
@@ -577,6 +583,10 @@ hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t
             k_contacts<false, true><<<cb, kContactThreads, 0, d.stream>>>(d, a);
         else
             k_contacts<false, false><<<cb, kContactThreads, 0, d.stream>>>(d, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = launch_contacts_ordered(d, a, groups, sensors);
+        if (e != hipSuccess) return e;
     }
     if (d.n_jcomp > 0 && joint_iterations > 0)
         k_joints<<<div_up(d.n_jcomp, 64), 64, 0, d.stream>>>(d, a, joint_iterations);

@@ -27,6 +27,9 @@ float col_inertia(const bg_collider& c) {
     return inertia + m * (d * d);
 }
 
+// Steps a single-sphere world stays on the general path after a push-out.
+constexpr uint32_t kPushOutHold = 4;
+
 template <class T>
 hipError_t dmalloc(T** p, size_t n) {
     *p = nullptr;
@@ -516,26 +519,47 @@ int World::set_event_capacity(uint32_t cap) {
 int World::collect_events() {
     DeviceWorld& d = d_;
     if (!d.allocated || !d.events) return BG_OK;
-    uint32_t n = 0;
-    HIP_TRY(hipMemcpyAsync(&n, &d.accum->event_count, sizeof(uint32_t), hipMemcpyDeviceToHost, d.stream));
+    unsigned long long n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, &d.accum->event_count, sizeof(n), hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
     if (n == 0) return BG_OK;
-    const uint32_t kept = n < d.event_cap ? n : d.event_cap;
-    events_dropped_ += n - kept;
+    const uint32_t kept = n < d.event_cap ? (uint32_t)n : d.event_cap;
     std::vector<DevEvent> ev;
     HIP_TRY(d2h(ev, d.events, kept, d.stream));
-    HIP_TRY(hipMemsetAsync(&d.accum->event_count, 0, sizeof(uint32_t), d.stream));
+    HIP_TRY(hipMemsetAsync(&d.accum->event_count, 0, sizeof(n), d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
+    // (substep, A, B) is the reference's send order (physics.rs:248-249, 304-311); a
+    // withdrawal record (pad 1, from the ordered contact pass) sorts right after the
+    // record it cancels
     std::sort(ev.begin(), ev.end(), [](const DevEvent& x, const DevEvent& y) {
         if (x.sub != y.sub) return x.sub < y.sub;
         if (x.a != y.a) return x.a < y.a;
-        return x.b < y.b;
+        if (x.b != y.b) return x.b < y.b;
+        return x.pad < y.pad;
     });
+    // Records are reserved substep after substep, so a full buffer holds every event of
+    // the substeps before the one that overflowed and an arbitrary part of that one:
+    // that substep's records are dropped too, so the kept set does not depend on the
+    // order of the device's atomics.
+    uint64_t dropped = n - kept;
+    size_t end = ev.size();
+    if (n > kept && end > 0) {
+        const uint32_t last = ev[end - 1].sub;
+        while (end > 0 && ev[end - 1].sub == last) --end;
+        for (size_t i = end; i < ev.size(); ++i) dropped += ev[i].pad ? 0u : 1u;
+    }
+    events_dropped_ += dropped;
     if (event_head_ > 0 && event_head_ == event_queue_.size()) {
         event_queue_.clear();
         event_head_ = 0;
     }
-    for (const DevEvent& e : ev) {
+    for (size_t i = 0; i < end; ++i) {
+        const DevEvent& e = ev[i];
+        if (e.pad) continue;  // a withdrawal whose record is not here
+        if (i + 1 < end && ev[i + 1].pad && ev[i + 1].sub == e.sub && ev[i + 1].a == e.a && ev[i + 1].b == e.b) {
+            ++i;  // withdrawn by the ordered contact pass
+            continue;
+        }
         bg_collision_event o;
         o.col_handle_a = colliders_.handle_at(col_slot_of_dense_[e.a]);
         o.col_handle_b = colliders_.handle_at(col_slot_of_dense_[e.b]);
@@ -679,6 +703,9 @@ void World::free_arrays() {
         dfree(d.con_r);
     dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.crec);
     dfree(d.grid), dfree(d.accum), dfree(d.block_contacts), dfree(d.events), dfree(d.vel_next);
+    dfree(d.ord_rows), dfree(d.ord_n), dfree(d.ord_mark), dfree(d.pos_pre), dfree(d.mv_key), dfree(d.mv_abs);
+    dfree(d.mv_list), dfree(d.ord_heap), dfree(d.ord_log), dfree(d.aff_list), dfree(d.aff_mark), dfree(d.ord_cand);
+    dfree(d.ord_terms);
     sort_temp_free(d.sort);
     hipStream_t s = d.stream;
     int dev = d.device;
@@ -691,6 +718,34 @@ void World::free_device() {
     free_arrays();
     if (d_.stream) (void)hipStreamDestroy(d_.stream);
     d_.stream = nullptr;
+}
+
+// Whether two colliders of the snapshot pass the reference predicate with
+// coincident centres (physics.rs:251-272: parents exist and differ, groups
+// test, dist < r_a + r_b and dist < 1e-6), i.e. whether the next contact pass
+// takes the push-out branch.  Same f32 arithmetic as the device; candidates
+// from a sort by x (coincident centres are within 1e-6 in x).
+static bool has_coincident_pair(const std::vector<float2>& abs, const std::vector<float>& radius,
+                                const std::vector<int32_t>& parent, const std::vector<uint2>& groups,
+                                bool use_groups) {
+    const size_t n = abs.size();
+    std::vector<uint32_t> idx;
+    idx.reserve(n);
+    for (uint32_t i = 0; i < n; ++i)
+        if (std::isfinite(abs[i].x) && std::isfinite(abs[i].y) && parent[i] >= 0) idx.push_back(i);
+    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return abs[a].x < abs[b].x; });
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const uint32_t a = idx[k];
+        for (size_t q = k + 1; q < idx.size() && abs[idx[q]].x - abs[a].x < 1e-6f; ++q) {
+            const uint32_t b = idx[q];
+            if (parent[a] == parent[b]) continue;
+            if (use_groups && !((groups[a].x & groups[b].y) != 0u && (groups[b].x & groups[a].y) != 0u)) continue;
+            const float dx = abs[a].x - abs[b].x, dy = abs[a].y - abs[b].y;
+            const float dist = sqrtf(dx * dx + dy * dy);
+            if (dist < radius[a] + radius[b] && dist < 1e-6f) return true;
+        }
+    }
+    return false;
 }
 
 // Flatten the arenas into dense SoA (slot order without holes = the
@@ -937,6 +992,14 @@ int World::upload() {
     bool sphere = (nb == nc) && ns == 0 && nj == 0 && nb > 0;
     for (uint32_t i = 0; sphere && i < nb; ++i)
         if (bc_start[i + 1] - bc_start[i] != 1 || bc_list[bc_start[i]] != i || cpar[i] != (int32_t)i) sphere = false;
+    // A push-out needs the ordered contact pass, which the general path carries: a
+    // single-sphere world takes that path while a pair of its snapshot coincides (or
+    // while push-outs kept happening in the last steps).
+    if (sphere && batch_start_.empty() && params_.collisions_enabled &&
+        (push_out_hold_ > 0 || has_coincident_pair(cabs, crad, cpar, cgrp, groups_))) {
+        sphere = false;
+        if (push_out_hold_ == 0) push_out_hold_ = kPushOutHold;
+    }
     path_ = sphere ? StepPath::Sphere : StepPath::General;
     if (sphere) {
         for (uint32_t i = 0; i < nb; ++i) {
@@ -1162,6 +1225,26 @@ int World::upload() {
         HIP_TRY(dmalloc(&d.cell_count, (size_t)d.cells_cap + 1));
         HIP_TRY(hipMemsetAsync(d.cell_count, 0, sizeof(uint32_t) * ((size_t)d.cells_cap + 1), s));
         HIP_TRY(dmalloc(&d.crec, nc));
+        // ordered contact pass (push-outs): idle state is no rows, no marks, no rewrites
+        const uint32_t ncw = std::max<uint32_t>(nc, 1), nbw = std::max<uint32_t>(nb, 1);
+        HIP_TRY(dmalloc(&d.ord_rows, ncw));
+        HIP_TRY(dmalloc(&d.ord_n, 1));
+        HIP_TRY(dmalloc(&d.ord_mark, (ncw + 31) / 32));
+        HIP_TRY(dmalloc(&d.pos_pre, nbw));
+        HIP_TRY(dmalloc(&d.mv_key, ncw));
+        HIP_TRY(dmalloc(&d.mv_abs, ncw));
+        HIP_TRY(dmalloc(&d.mv_list, ncw));
+        HIP_TRY(dmalloc(&d.ord_heap, ncw));
+        d.ord_log_cap = std::min<uint32_t>(std::max<uint32_t>(2u * ncw, 16384u), 1u << 21);
+        HIP_TRY(dmalloc(&d.ord_log, d.ord_log_cap));
+        HIP_TRY(dmalloc(&d.aff_list, nbw));
+        HIP_TRY(dmalloc(&d.aff_mark, (nbw + 31) / 32));
+        HIP_TRY(dmalloc(&d.ord_cand, kOrdCand));
+        HIP_TRY(dmalloc(&d.ord_terms, kOrdTerms));
+        HIP_TRY(hipMemsetAsync(d.ord_n, 0, sizeof(uint32_t), s));
+        HIP_TRY(hipMemsetAsync(d.ord_mark, 0, sizeof(uint32_t) * ((ncw + 31) / 32), s));
+        HIP_TRY(hipMemsetAsync(d.aff_mark, 0, sizeof(uint32_t) * ((nbw + 31) / 32), s));
+        HIP_TRY(hipMemsetAsync(d.mv_key, 0xFF, sizeof(unsigned long long) * ncw, s));
     }
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
@@ -1382,12 +1465,13 @@ int World::run_substep(float h, bool fresh, bool last) {
     return stage_end(BG_STAGE_SUBSTEP);
 }
 
-int World::check_errors() {
+int World::check_errors(uint32_t* flags_out) {
     DeviceWorld& d = d_;
     StepAccum acc;
     HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
-    uint32_t panic_bits = acc.error_flags & (ERR_NAN_NORMAL | ERR_JOINT_MASS | ERR_JOINT_ROT);
+    if (flags_out) *flags_out = acc.error_flags;
+    uint32_t panic_bits = acc.error_flags & (ERR_NAN_NORMAL | ERR_JOINT_MASS | ERR_JOINT_ROT | ERR_ORDERED_CAP);
     if (acc.error_flags) {
         uint32_t zero = 0;
         HIP_TRY(hipMemcpyAsync(&d.accum->error_flags, &zero, 4, hipMemcpyHostToDevice, d.stream));
@@ -1398,6 +1482,8 @@ int World::check_errors() {
         return fail(BG_ERR_PANIC, "assertion failed: body calculated_mass > 0.0 (physics.rs:447-448)");
     if (panic_bits & ERR_JOINT_ROT)
         return fail(BG_ERR_PANIC, "assertion failed: body rotation is finite (physics.rs:471-474)");
+    if (panic_bits & ERR_ORDERED_CAP)
+        return fail(BG_ERR_CAPACITY, "a push-out cluster (physics.rs:272-286) outgrew the ordered contact pass's buffers");
     return BG_OK;
 }
 
@@ -1453,7 +1539,28 @@ int World::step(double delta, uint32_t n) {
     if (st) return st;
     advance_time(delta, n);
     host_stale_ = true;
-    return check_errors();
+    uint32_t flags = 0;
+    st = check_errors(&flags);
+    if (st) return st;
+    route_after_step(flags);
+    return BG_OK;
+}
+
+// Path of the next steps of a single-sphere world (see push_out_hold_).  A
+// switch re-uploads from the host mirror, which carries the whole state.
+void World::route_after_step(uint32_t flags) {
+    if (!batch_start_.empty()) return;
+    bool to_general = false, to_sphere = false;
+    if (flags & INFO_PUSH_OUT) {
+        push_out_hold_ = kPushOutHold;
+        if (path_ == StepPath::Sphere) {
+            ++pairwise_push_out_steps_;
+            to_general = true;
+        }
+    } else if (push_out_hold_ > 0 && --push_out_hold_ == 0 && path_ == StepPath::General) {
+        to_sphere = sphere_eligible();
+    }
+    if ((to_general || to_sphere) && sync_host() == BG_OK) mark_dirty();
 }
 
 // Physics::fixed_step, physics.rs:84-99.

@@ -128,7 +128,8 @@ private:
     void advance_time(double delta, uint32_t n);
     bool sphere_eligible() const;
     void dense_slots(std::vector<uint32_t>& bodies, std::vector<uint32_t>& cols) const;
-    int check_errors();
+    int check_errors(uint32_t* flags_out = nullptr);
+    void route_after_step(uint32_t flags);
     int device_contacts(uint64_t* out);
     void free_device();
     void free_arrays();
@@ -165,6 +166,11 @@ private:
     bool groups_ = false, sensors_ = false, zero_off_ = false, use_drift_ = false;
     bool structure_error_ = false;
     std::string structure_msg_;
+    // push-outs (physics.rs:272-286) are replayed in order on the general path only: a
+    // single-sphere world steps there while its uploaded snapshot has coincident pairs
+    // and for push_out_hold_ more steps after a step that had a push-out
+    uint32_t push_out_hold_ = 0;
+    uint64_t pairwise_push_out_steps_ = 0;  // fused-path steps that resolved push-outs pair by pair
     // dense <-> slot maps of the last upload
     std::vector<uint32_t> body_slot_of_dense_, col_slot_of_dense_;
     std::vector<int32_t> body_dense_of_slot_, col_dense_of_slot_;

@@ -83,15 +83,12 @@ def test_events_overflow_are_counted_and_kept_events_are_the_reference_s():
     cap = 400
     assert cols.shape[0] > 2 * cap
     p, hs, vs, dropped = gpu_events(sc, 5, cap=cap)
-    assert hs.shape[0] == cap and dropped == cols.shape[0] - cap
-    # a pair collides again in later substeps with other velocities: key on both
-    pos = {}
-    for k, (ab, v) in enumerate(zip(cols, bits(vels))):
-        pos.setdefault((int(ab[0]), int(ab[1])) + tuple(int(x) for x in v), k)
-    got = slots(hs)
-    order = [pos.get((int(ab[0]), int(ab[1])) + tuple(int(x) for x in v), -1) for ab, v in zip(got, bits(vs))]
-    assert min(order) >= 0, "every kept event is one of the reference's"
-    assert order == sorted(order), "the drain is in reference order even when partial"
+    # a full buffer keeps every event of the substeps before the one that overflowed
+    # (that one's are dropped whole, so the kept set does not depend on atomic order):
+    # a prefix of the reference's stream, and the rest is counted
+    k = hs.shape[0]
+    assert k <= cap and dropped == cols.shape[0] - k
+    assert_same_events(hs, vs, cols[:k], vels[:k])
     # after a drain the buffer is empty again
     p.step(sc.dt, 1)
     hs2, _, dropped2 = p.drain_events()

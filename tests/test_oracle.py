@@ -196,6 +196,66 @@ def test_push_out_coincident_centres():
     assert np.allclose(o.pos, [[0.5, 2.0], [1.5, 2.0]], atol=1e-12)
 
 
+def test_push_out_later_pairs_read_the_rewritten_snapshot():
+    """Three coincident equal spheres (r = 0.1) at the origin, worked by hand
+    through physics.rs:248-300 in (i, j<i) order:
+      (1,0): push-out -> x1 = 0.01, x0 = -0.01; both snapshots are rewritten to
+             those live positions; axis 0.02, delta 0.18, split 0.09 each
+             -> x1 = 0.1, x0 = -0.1;
+      (2,0): snapshot 0 now reads -0.01: axis 0.01, delta 0.19 -> x2 = 0.095,
+             x0 = -0.195;
+      (2,1): snapshot 1 reads 0.01: axis -0.01 -> x2 = 0, x1 = 0.195.
+    A per-pair push-out (every pair +-0.01 from the unchanged snapshot) gives
+    (-0.2, 0, 0.2) instead."""
+    sc = one_body_scene([[0, 0], [0, 0], [0, 0]], radius=0.1, gravity=(0.0, 0.0))
+    o = OracleWorld(sc, np.float64)
+    o.contact_pairs()
+    assert np.allclose(o.pos, [[-0.195, 0.0], [0.195, 0.0], [0.0, 0.0]], atol=1e-12)
+    assert o.w.push_outs == 1
+
+
+def test_push_out_rewrites_with_the_unrotated_offset():
+    """physics.rs:280-283 rebuilds a pushed collider's snapshot as
+    position + offset.translation, without the body's rotation.  Body 0 sits
+    at the origin rotated pi/2 with its collider offset (1, 0), so its
+    snapshot is (0, 1) -- on top of body 1's collider.  Pair (1, 0), by hand:
+    push -> body 1 at (0.01, 1), body 0 at (-0.01, 0); snapshots (0.01, 1) and
+    (-0.01, 0) + (1, 0) = (0.99, 0); axis (-0.98, 1), rs = 2, equal masses
+    split delta in half.  The rotated offset would give axis (0.02, 0)."""
+    sc = scenes.Scene(
+        pos=np.array([[0, 0], [0, 1]], np.float32), vreq=np.zeros((2, 2), np.float32),
+        has_vreq=np.zeros(2, np.uint8), gravity_mod=np.ones(2, np.float32), body_type=np.zeros(2, np.uint8),
+        rotation=np.array([math.pi / 2, 0.0], np.float32), col_parent=np.array([0, 1], np.int32),
+        col_off=np.array([[1, 0], [0, 0]], np.float32), col_abs=np.array([[0, 1], [0, 1]], np.float32),
+        col_r=np.array([1.0, 1.0], np.float32), col_mass_override=np.zeros(2, np.float32),
+        col_has_mo=np.zeros(2, np.uint8), col_sensor=np.zeros(2, np.uint8), col_mem=np.full(2, 0xFFFFFFFF, np.uint32),
+        col_filt=np.full(2, 0xFFFFFFFF, np.uint32), gravity=(0.0, 0.0), dt=1 / 60)
+    o = OracleWorld(sc, np.float64)
+    o.contact_pairs()
+    axis = np.array([-0.98, 1.0])
+    dist = math.hypot(*axis)
+    half = 0.5 * (2.0 - dist) * axis / dist
+    assert np.allclose(o.pos[1], np.array([0.01, 1.0]) + half, atol=1e-12)
+    assert np.allclose(o.pos[0], np.array([-0.01, 0.0]) - half, atol=1e-12)
+
+
+def test_push_out_cluster_grid_equals_bruteforce():
+    """The grid variant of the oracle's contact pass re-bins pushed colliders
+    and re-queries the row; on 20 coincident spheres inside a crowd it visits
+    the reference loop's exact sequence (bit-identical f32 positions)."""
+    rng = np.random.default_rng(7)
+    crowd = rng.uniform(-1.5, 1.5, (80, 2))
+    pos = np.concatenate([crowd[:30], np.full((20, 2), 0.25), crowd[30:]]).astype(np.float32)
+    rad = rng.uniform(0.05, 0.2, 100).astype(np.float32)
+    sc = one_body_scene(pos, radius=rad, gravity=(0.0, 0.0))
+    a = OracleWorld(sc, np.float32, use_grid=True)
+    b = OracleWorld(sc, np.float32, use_grid=False)
+    pa, pb = a.contact_pairs(), b.contact_pairs()
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(a.pos.view(np.uint32), b.pos.view(np.uint32))
+    assert a.w.push_outs == b.w.push_outs > 1
+
+
 # --------------------------------------------------------------- springs
 
 def test_spring_two_body_oscillator_frequency():
