@@ -68,7 +68,9 @@ def cpu_baseline(scene, budget_s: float) -> dict:
         "sample": (f"C restatement of blobs Physics::step (oracle/, f32, O(C^2) brute-force contacts as in "
                    f"physics.rs:248-249) on the same {n}-sphere scene: {rows} contact-loop rows "
                    f"({tests:.3g} pair tests, {t_rows:.1f} s) extrapolated to {substeps} x C(C-1)/2 pair tests per "
-                   f"step, plus one full step with collisions off ({t_rest:.2f} s); single thread; extrapolated"),
+                   f"step, plus one full step with collisions off ({t_rest:.2f} s); single thread; extrapolated. "
+                   f"Not included: the reference's per-body SpatialHash move (physics.rs:341) and per-contact "
+                   f"channel send (physics.rs:304-311), which would only add to its time"),
         "s_per_step": t_step,
     }
 
@@ -169,6 +171,31 @@ def differentiable_leg(n_worlds: int, n_steps: int, ck_every: int) -> dict:
     return rec
 
 
+def plain_small_leg(steps: int = 600) -> dict:
+    """The plain CPU-runnable scene (1,024 single spheres + static walls,
+    step(1/60) x 600) as a drop-in caller drives it: one Physics::step per
+    call.  Synchronous mode waits for each step and checks its panic flags;
+    deferred mode (bg_set_deferred_checks) returns once the step is queued."""
+    from blobs_amd import scenes
+    sc = scenes.plain_small_scene(steps=steps)
+    rec = {"workload": "plain_small: 1024 spheres + static walls, step(1/60) x 600, one step per call",
+           "bodies": sc.n_bodies, "steps": steps, "unit": "ms/step"}
+    for mode in ("synchronous", "deferred"):
+        p, _ = sc.build_physics()
+        if mode == "deferred":
+            p.set_deferred_checks(True)
+        p.step(sc.dt)
+        p.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            p.step(sc.dt)
+        p.synchronize()
+        rec[f"ms_per_step_{mode}"] = 1000.0 * (time.perf_counter() - t0) / (steps - 1)
+    log(f"plain small: {rec['ms_per_step_synchronous']:.3f} ms/step synchronous, "
+        f"{rec['ms_per_step_deferred']:.3f} ms/step deferred")
+    return rec
+
+
 def multi_collider_leg(steps: int = 20, warmup: int = 3) -> dict:
     """The multi-collider scene on the general path: 65,536 bodies with 1-8
     sphere colliders each, 32,768 springs, 16,384 fixed joints, step(1/60)."""
@@ -207,6 +234,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-differentiable", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-chunks", type=int, default=85,
                     help="timed 10-step chunks (with warm-up and tuning the run stays within 1000 steps)")
@@ -330,12 +358,17 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_per_launch(sc.n_bodies),
+            "traffic_source": "committed rocprofv3 --pmc summary of this kernel and workload "
+                              "(profiles/k_sphere_substep_plain_1M_traffic.json: FETCH_SIZE and WRITE_SIZE per "
+                              "dispatch, each calibrated by a 256 MiB copy kernel), not measured in this run",
             "bytes_per_launch": BYTES_PER_BODY_SUBSTEP * sc.n_bodies,
             "avg_launch_us": avg_sub_ms * 1e3,
             "rebuild_avg_us": ms_reb / max(n_reb, 1) * 1e3,
             "rebuilds": n_reb,
-            "launch_timing": "HIP events around every launch, on the world's stream, over the same steps of a "
-                             "fresh copy of the world (the timed region itself runs without them)",
+            "launch_timing": "the kernel's own start and end timestamps (hipExtLaunchKernelGGL events on the "
+                             "world's stream) for every substep launch over the same steps of a fresh copy of the "
+                             "world (the timed region itself runs without them); rocprofv3 --kernel-trace on the same "
+                             "command gives the same per-launch average",
             "instrumented_ms_per_step": 1000.0 * elapsed_instrumented / args.steps,
         },
         "cpu_baseline": None,
@@ -348,6 +381,8 @@ def main() -> int:
         line["differentiable"] = differentiable_leg(256, 200, 20)
     if not args.no_multi:
         line["multi_collider"] = multi_collider_leg()
+    if not args.no_small:
+        line["plain_small"] = plain_small_leg()
     if not args.no_cpu:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)

@@ -180,6 +180,7 @@ SIGNATURES = {
     "bg_set_broadphase": [WORLD, C.c_float, C.c_float, C.c_uint32],
     "bg_step_path": [WORLD, u32p],
     "bg_set_profiling": [WORLD, C.c_int],
+    "bg_set_deferred_checks": [WORLD, C.c_int],
     "bg_stage_stats": [WORLD, C.c_uint32, u64p, C.POINTER(C.c_double)],
     "bg_batch_run": [WORLD, C.c_double, C.c_uint32, C.c_uint32],
     "bg_batch_create": [C.POINTER(WORLD), C.c_uint32, C.POINTER(WORLD)],
@@ -200,6 +201,7 @@ SIGNATURES.update({
     "bg_rollout_forward": [ROLLOUT, C.c_double, C.c_uint32, f32p],
     "bg_rollout_backward": [ROLLOUT, f32p, f32p, f32p],
     "bg_rollout_final_positions": [ROLLOUT, f32p],
+    "bg_rollout_nonfinite": [ROLLOUT, C.POINTER(C.c_uint8)],
 })
 
 # entry points bound only when present in the loaded library

@@ -192,7 +192,7 @@ int World::step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks) {
         if (st) return st;
         advance_time(delta, chunk_steps * n_chunks);
         host_stale_ = true;
-        return check_errors();
+        return check_and_process();
     }
     // the first chunk after an upload runs eagerly (user inputs are consumed once)
     if (fresh_) {
@@ -266,7 +266,7 @@ int World::step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks) {
     if (e != hipSuccess && result == BG_OK) result = fail(BG_ERR_HIP, hipGetErrorString(e));
     host_stale_ = true;
     if (result != BG_OK) return result;
-    return check_errors();
+    return check_and_process();
 }
 
 int World::autotune(double delta, uint32_t steps_per_trial, float* cell_scale, float* skin, uint32_t* interval) {
@@ -324,7 +324,7 @@ int World::autotune(double delta, uint32_t steps_per_trial, float* cell_scale, f
     if (cell_scale) *cell_scale = best.cs;
     if (skin) *skin = best.sk;
     if (interval) *interval = best.iv;
-    return check_errors();
+    return check_and_process();
 }
 
 }  // namespace bg

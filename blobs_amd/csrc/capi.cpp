@@ -369,9 +369,17 @@ int bg_rollout_backward(bg_rollout* r, const float* dloss_dfinal_position, float
     });
 }
 
+int bg_rollout_nonfinite(bg_rollout* r, uint8_t* per_world) {
+    return rguard(r, [&](bg::Rollout& x) { return x.nonfinite(per_world); });
+}
+
 int bg_rollout_final_positions(bg_rollout* r, float* out) {
     if (!out) return BG_ERR_INVALID_ARGUMENT;
     return rguard(r, [&](bg::Rollout& x) { return x.final_positions(out); });
+}
+
+int bg_set_deferred_checks(bg_world* w, int enable) {
+    return guarded(w, [&](bg::World& x) { return x.set_deferred_checks(enable != 0); });
 }
 
 int bg_set_profiling(bg_world* w, int enable) {

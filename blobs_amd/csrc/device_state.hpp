@@ -189,6 +189,9 @@ struct DeviceWorld {
     hipStream_t stream = nullptr;
     int device = -1;
     bool allocated = false;
+    // stage timing of the fused substep: when set, its launch records these events
+    // at the kernel's own start and end (hipExtLaunchKernelGGL)
+    hipEvent_t prof_start = nullptr, prof_stop = nullptr;
 };
 
 // Scalars of one substep launch.

@@ -3,6 +3,8 @@
 // segment-by-segment adjoint.
 #include "diff.hpp"
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -190,6 +192,13 @@ int Rollout::create(World* const* worlds, uint32_t n, const float* targets, uint
     nb_ = (uint32_t)mass_.size();
     ns_ = (uint32_t)k_.size();
     n_worlds_ = n;
+    body0_.resize(n + 1);
+    for (uint32_t w = 0; w < n; ++w) body0_[w] = ws[w].b0;
+    body0_[n] = nb_;
+    spring_world_.resize(ns_);
+    for (uint32_t w = 0; w < n; ++w)
+        for (uint32_t s = ws[w].s0; s < ws[w].s0 + ws[w].ns; ++s) spring_world_[s] = w;
+    nonfinite_.assign(n, 0);
     // per body: its springs in ascending index (summation order of physics.rs:413 + springs.rs:25-47)
     std::vector<uint32_t> inc_start(nb_ + 1, 0), inc_list(2 * (size_t)ns_);
     for (uint32_t s = 0; s < ns_; ++s) {
@@ -355,10 +364,27 @@ int Rollout::backward(const float* dl_dfinal, float* grad_vreq, float* grad_k) {
         const uint32_t steps = (n_steps_ - first) < ck_every_ ? (n_steps_ - first) : ck_every_;
         HIP_TRY(launch_diff_backward_segment(a, seg, first, steps, seg == n_seg - 1, stream_));
     }
-    if (grad_vreq)
-        HIP_TRY(hipMemcpyAsync(grad_vreq, d_gv_, sizeof(float2) * nb_, hipMemcpyDeviceToHost, stream_));
-    if (grad_k && ns_) HIP_TRY(hipMemcpyAsync(grad_k, d_gk_, sizeof(float) * ns_, hipMemcpyDeviceToHost, stream_));
+    std::vector<float2> gv(nb_);
+    std::vector<float> gk(ns_);
+    HIP_TRY(hipMemcpyAsync(gv.data(), d_gv_, sizeof(float2) * nb_, hipMemcpyDeviceToHost, stream_));
+    if (ns_) HIP_TRY(hipMemcpyAsync(gk.data(), d_gk_, sizeof(float) * ns_, hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
+    // worlds whose gradients left the f32 range (reported, not an error: the fixed-pair
+    // derivative of the contact projection grows with the horizon)
+    nonfinite_.assign(n_worlds_, 0);
+    for (uint32_t w = 0; w < n_worlds_; ++w)
+        for (uint32_t i = body0_[w]; i < body0_[w + 1] && !nonfinite_[w]; ++i)
+            if (!std::isfinite(gv[i].x) || !std::isfinite(gv[i].y)) nonfinite_[w] = 1;
+    for (uint32_t s = 0; s < ns_; ++s)
+        if (!std::isfinite(gk[s])) nonfinite_[spring_world_[s]] = 1;
+    if (grad_vreq) memcpy(grad_vreq, gv.data(), sizeof(float2) * nb_);
+    if (grad_k && ns_) memcpy(grad_k, gk.data(), sizeof(float) * ns_);
+    return BG_OK;
+}
+
+int Rollout::nonfinite(uint8_t* per_world) const {
+    if (!per_world) return BG_ERR_INVALID_ARGUMENT;
+    for (uint32_t w = 0; w < n_worlds_; ++w) per_world[w] = w < nonfinite_.size() ? nonfinite_[w] : 0;
     return BG_OK;
 }
 

@@ -90,6 +90,7 @@ public:
     int forward(double delta, uint32_t n_steps, float* loss_per_world);
     int backward(const float* dl_dfinal, float* grad_vreq, float* grad_k);
     int final_positions(float* out);
+    int nonfinite(uint8_t* per_world) const;
     uint32_t n_bodies() const { return nb_; }
     uint32_t n_springs() const { return ns_; }
     uint32_t n_worlds() const { return n_worlds_; }
@@ -109,6 +110,9 @@ private:
     uint32_t n_ck_alloc_ = 0, tape_sub_alloc_ = 0;
     float gx_ = 0.0f, gy_ = 0.0f, h_ = 0.0f;
     bool collisions_ = true, damped_ = false, have_forward_ = false;
+    std::vector<uint8_t> nonfinite_;  // per world, after the last backward: a gradient entry is inf or NaN
+    std::vector<uint32_t> body0_;     // first body of each world (n_worlds + 1 entries)
+    std::vector<uint32_t> spring_world_;  // world of each spring
     std::vector<float2> p0_, q0_, v0_, vreq_, acc0_;
     std::vector<float> mass_, radius_, gmod_, rest_, k_, c_;
     std::vector<uint32_t> flags_;

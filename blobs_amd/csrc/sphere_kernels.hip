@@ -28,6 +28,8 @@
 // with radius r + r_max, so every pair of the reference predicate is
 // visited exactly once from each side, whatever the bodies do; skin, cell
 // size and rebuild interval change only the speed.
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 #include "device_state.hpp"
 #include "step_common.hpp"
@@ -997,7 +999,14 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.nbr = d.nbr;
     k.nbr_len = d.nbr_len;
     const uint32_t blocks = div_up(d.nb, kSubstepThreads);
-#define BG_SPHERE(G, S, Z, C, P, B) k_sphere_substep<G, S, Z, C, P, B><<<blocks, kSubstepThreads, 0, d.stream>>>(k, a)
+#define BG_SPHERE(G, S, Z, C, P, B)                                                                        \
+    do {                                                                                                  \
+        if (d.prof_start)                                                                                 \
+            hipExtLaunchKernelGGL(k_sphere_substep<G, S, Z, C, P, B>, dim3(blocks), dim3(kSubstepThreads), 0, \
+                                  d.stream, d.prof_start, d.prof_stop, 0, k, a);                          \
+        else                                                                                              \
+            k_sphere_substep<G, S, Z, C, P, B><<<blocks, kSubstepThreads, 0, d.stream>>>(k, a);         \
+    } while (0)
 #define BG_SPHERE_PB(G, S, Z, C)                 \
     do {                                         \
         if (gather && batched)                   \

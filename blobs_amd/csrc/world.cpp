@@ -85,6 +85,8 @@ World::World(float gx, float gy, bool use_spatial_hash) {
 
 World::~World() {
     if (d_.stream) (void)hipStreamSynchronize(d_.stream);
+    if (check_event_) (void)hipEventDestroy(check_event_);
+    if (host_acc_) (void)hipHostFree(host_acc_);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     for (auto& v : pending_)
         for (auto& pr : v) {
@@ -142,15 +144,19 @@ int World::stage_stats(uint32_t stage, uint64_t* launches, double* total_ms) {
     return BG_OK;
 }
 
+hipError_t World::pooled_event(hipEvent_t* out) {
+    if (!event_pool_.empty()) {
+        *out = event_pool_.back();
+        event_pool_.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(out);
+}
+
 int World::stage_begin(uint32_t stage) {
     if (!profiling_) return BG_OK;
     hipEvent_t e;
-    if (!event_pool_.empty()) {
-        e = event_pool_.back();
-        event_pool_.pop_back();
-    } else {
-        HIP_TRY(hipEventCreate(&e));
-    }
+    HIP_TRY(pooled_event(&e));
     HIP_TRY(hipEventRecord(e, d_.stream));
     open_[stage] = e;
     return BG_OK;
@@ -159,12 +165,7 @@ int World::stage_begin(uint32_t stage) {
 int World::stage_end(uint32_t stage) {
     if (!profiling_ || !open_[stage]) return BG_OK;
     hipEvent_t e;
-    if (!event_pool_.empty()) {
-        e = event_pool_.back();
-        event_pool_.pop_back();
-    } else {
-        HIP_TRY(hipEventCreate(&e));
-    }
+    HIP_TRY(pooled_event(&e));
     HIP_TRY(hipEventRecord(e, d_.stream));
     pending_[stage].push_back({open_[stage], e});
     open_[stage] = nullptr;
@@ -1456,13 +1457,67 @@ int World::run_substep(float h, bool fresh, bool last) {
     a.write_vel = (last || d.events) ? 1u : 0u;
     a.t_prev = (float)((double)(a.sub - 1) * (double)h);
     a.t_cur = (float)((double)a.sub * (double)h);
-    if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
     if (path_ == StepPath::Sphere) {
-        HIP_TRY(launch_sphere_substep(d, a, groups_, sensors_, zero_off_, d.nk > 0));
-    } else {
-        HIP_TRY(launch_general_substep(d, a, params_.joint_iterations, groups_, sensors_));
+        // the fused substep is one kernel: its launch records the stage events at the
+        // kernel's own start and end, so the stage time is the kernel's duration
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (profiling_) {
+            HIP_TRY(pooled_event(&e0));
+            HIP_TRY(pooled_event(&e1));
+        }
+        d.prof_start = e0;
+        d.prof_stop = e1;
+        const hipError_t e = launch_sphere_substep(d, a, groups_, sensors_, zero_off_, d.nk > 0);
+        d.prof_start = d.prof_stop = nullptr;
+        HIP_TRY(e);
+        if (profiling_) pending_[BG_STAGE_SUBSTEP].push_back({e0, e1});
+        return BG_OK;
     }
+    if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
+    HIP_TRY(launch_general_substep(d, a, params_.joint_iterations, groups_, sensors_));
     return stage_end(BG_STAGE_SUBSTEP);
+}
+
+// Panics and routing from the step record's error flags (physics.rs:293,
+// 447-448, 471-474).
+int World::process_flags(uint32_t flags) {
+    const uint32_t panic_bits = flags & (ERR_NAN_NORMAL | ERR_JOINT_MASS | ERR_JOINT_ROT | ERR_ORDERED_CAP);
+    route_after_step(flags);
+    if (panic_bits & ERR_NAN_NORMAL) return fail(BG_ERR_PANIC, "assertion failed: !n.is_nan() (physics.rs:293)");
+    if (panic_bits & ERR_JOINT_MASS)
+        return fail(BG_ERR_PANIC, "assertion failed: body calculated_mass > 0.0 (physics.rs:447-448)");
+    if (panic_bits & ERR_JOINT_ROT)
+        return fail(BG_ERR_PANIC, "assertion failed: body rotation is finite (physics.rs:471-474)");
+    if (panic_bits & ERR_ORDERED_CAP)
+        return fail(BG_ERR_CAPACITY, "a push-out cluster (physics.rs:272-286) outgrew the ordered contact pass's buffers");
+    return BG_OK;
+}
+
+// Deferred checks: the step record's flags come back through pinned memory
+// behind the step's kernels, and are read by a later call once they arrived.
+int World::set_deferred_checks(bool on) {
+    if (!on) {
+        int st = finish_deferred_check(true);
+        deferred_checks_ = false;
+        return st;
+    }
+    if (!host_acc_) HIP_TRY(hipHostMalloc((void**)&host_acc_, sizeof(StepAccum), hipHostMallocDefault));
+    if (!check_event_) HIP_TRY(hipEventCreateWithFlags(&check_event_, hipEventDisableTiming));
+    deferred_checks_ = true;
+    return BG_OK;
+}
+
+int World::finish_deferred_check(bool wait) {
+    if (!check_pending_) return BG_OK;
+    if (wait) {
+        HIP_TRY(hipEventSynchronize(check_event_));
+    } else {
+        const hipError_t q = hipEventQuery(check_event_);
+        if (q == hipErrorNotReady) return BG_OK;
+        HIP_TRY(q);
+    }
+    check_pending_ = false;
+    return process_flags(host_acc_->error_flags);
 }
 
 int World::check_errors(uint32_t* flags_out) {
@@ -1471,19 +1526,11 @@ int World::check_errors(uint32_t* flags_out) {
     HIP_TRY(hipMemcpyAsync(&acc, d.accum, sizeof(acc), hipMemcpyDeviceToHost, d.stream));
     HIP_TRY(hipStreamSynchronize(d.stream));
     if (flags_out) *flags_out = acc.error_flags;
-    uint32_t panic_bits = acc.error_flags & (ERR_NAN_NORMAL | ERR_JOINT_MASS | ERR_JOINT_ROT | ERR_ORDERED_CAP);
     if (acc.error_flags) {
         uint32_t zero = 0;
         HIP_TRY(hipMemcpyAsync(&d.accum->error_flags, &zero, 4, hipMemcpyHostToDevice, d.stream));
         HIP_TRY(hipStreamSynchronize(d.stream));
     }
-    if (panic_bits & ERR_NAN_NORMAL) return fail(BG_ERR_PANIC, "assertion failed: !n.is_nan() (physics.rs:293)");
-    if (panic_bits & ERR_JOINT_MASS)
-        return fail(BG_ERR_PANIC, "assertion failed: body calculated_mass > 0.0 (physics.rs:447-448)");
-    if (panic_bits & ERR_JOINT_ROT)
-        return fail(BG_ERR_PANIC, "assertion failed: body rotation is finite (physics.rs:471-474)");
-    if (panic_bits & ERR_ORDERED_CAP)
-        return fail(BG_ERR_CAPACITY, "a push-out cluster (physics.rs:272-286) outgrew the ordered contact pass's buffers");
     return BG_OK;
 }
 
@@ -1533,17 +1580,29 @@ void World::advance_time(double delta, uint32_t n) {
 
 int World::step(double delta, uint32_t n) {
     if (n == 0) return BG_OK;
-    int st = prepare_step();
+    int st = finish_deferred_check(false);  // a panic of an earlier step surfaces here
+    if (st) return st;
+    st = prepare_step();
     if (st) return st;
     st = launch_steps(delta, n);
     if (st) return st;
     advance_time(delta, n);
     host_stale_ = true;
+    if (deferred_checks_) {
+        // wait for the previous record before its pinned buffer is reused
+        st = finish_deferred_check(true);
+        if (st) return st;
+        DeviceWorld& d = d_;
+        HIP_TRY(hipMemcpyAsync(host_acc_, d.accum, sizeof(StepAccum), hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(launch_fill_u32(&d.accum->error_flags, 0u, 1, d.stream));
+        HIP_TRY(hipEventRecord(check_event_, d.stream));
+        check_pending_ = true;
+        return BG_OK;
+    }
     uint32_t flags = 0;
     st = check_errors(&flags);
     if (st) return st;
-    route_after_step(flags);
-    return BG_OK;
+    return process_flags(flags);
 }
 
 // Path of the next steps of a single-sphere world (see push_out_hold_).  A
@@ -1579,7 +1638,7 @@ int World::fixed_step(double frame_time) {
 
 int World::synchronize() {
     if (d_.stream) HIP_TRY(hipStreamSynchronize(d_.stream));
-    return BG_OK;
+    return finish_deferred_check(true);
 }
 
 }  // namespace bg

@@ -102,6 +102,7 @@ public:
     int debug_data(bg_affine2* bodies, bg_affine2* cols, float* radius, float* joints, float* springs,
                    uint32_t* counts);
     int set_profiling(bool enable);
+    int set_deferred_checks(bool on);
     int debug_accum(uint32_t* out4);
     int debug_list_lengths(uint32_t* out);
     int stage_stats(uint32_t stage, uint64_t* launches, double* total_ms);
@@ -129,10 +130,18 @@ private:
     bool sphere_eligible() const;
     void dense_slots(std::vector<uint32_t>& bodies, std::vector<uint32_t>& cols) const;
     int check_errors(uint32_t* flags_out = nullptr);
+    int process_flags(uint32_t flags);
+    int check_and_process() {
+        uint32_t f = 0;
+        int st = check_errors(&f);
+        return st ? st : process_flags(f);
+    }
+    int finish_deferred_check(bool wait);
     void route_after_step(uint32_t flags);
     int device_contacts(uint64_t* out);
     void free_device();
     void free_arrays();
+    hipError_t pooled_event(hipEvent_t* out);
     int stage_begin(uint32_t stage);
     int stage_end(uint32_t stage);
     int collect_stage_times();
@@ -170,6 +179,10 @@ private:
     // single-sphere world steps there while its uploaded snapshot has coincident pairs
     // and for push_out_hold_ more steps after a step that had a push-out
     uint32_t push_out_hold_ = 0;
+    // deferred checks: the step record's flags arrive in pinned host memory
+    bool deferred_checks_ = false, check_pending_ = false;
+    StepAccum* host_acc_ = nullptr;
+    hipEvent_t check_event_ = nullptr;
     uint64_t pairwise_push_out_steps_ = 0;  // fused-path steps that resolved push-outs pair by pair
     // dense <-> slot maps of the last upload
     std::vector<uint32_t> body_slot_of_dense_, col_slot_of_dense_;

@@ -441,6 +441,14 @@ class Physics:
     def step(self, delta: float, n_steps: int = 1) -> None:
         self._chk(self._lib.bg_step(self._w, float(delta), int(n_steps)))
 
+    def set_deferred_checks(self, enable: bool) -> None:
+        """step() returns once its kernels are queued; a panic surfaces at a
+        later step() or synchronize()."""
+        self._chk(self._lib.bg_set_deferred_checks(self._w, int(bool(enable))))
+
+    def synchronize(self) -> None:
+        self._chk(self._lib.bg_synchronize(self._w))
+
     def fixed_step(self, frame_time: float) -> None:
         self._chk(self._lib.bg_fixed_step(self._w, float(frame_time)))
 

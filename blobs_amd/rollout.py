@@ -39,6 +39,8 @@ class Rollout:
         nw, nb, ns = C.c_uint32(), C.c_uint32(), C.c_uint32()
         self._chk(self._lib.bg_rollout_sizes(r, C.byref(nw), C.byref(nb), C.byref(ns)))
         self.n_worlds, self.n_bodies, self.n_springs = nw.value, nb.value, ns.value
+        self.generation = 0  # forwards run so far (a backward belongs to the latest)
+        self.nonfinite_worlds = np.zeros(0, np.int64)
 
     def __del__(self):
         r = getattr(self, "_r", None)
@@ -63,6 +65,7 @@ class Rollout:
         """Runs the rollout; returns the loss of every world."""
         loss = np.zeros(self.n_worlds, np.float32)
         self._chk(self._lib.bg_rollout_forward(self._r, float(delta), int(n_steps), _fp(loss)))
+        self.generation += 1
         return loss
 
     def final_positions(self) -> np.ndarray:
@@ -71,18 +74,27 @@ class Rollout:
         return out
 
     def backward(self, dloss_dfinal=None):
-        """(d loss / d velocity_request [n_bodies, 2], d loss / d stiffness [n_springs])."""
+        """(d loss / d velocity_request [n_bodies, 2], d loss / d stiffness [n_springs]).
+        Worlds whose gradients hold inf or NaN (the fixed-pair derivative of
+        the contact projection can outgrow f32 over long horizons) are listed
+        in `nonfinite_worlds` after the call."""
         gv = np.zeros((self.n_bodies, 2), np.float32)
         gk = np.zeros(self.n_springs, np.float32)
         dl = None if dloss_dfinal is None else np.ascontiguousarray(dloss_dfinal, np.float32)
         self._chk(self._lib.bg_rollout_backward(self._r, None if dl is None else _fp(dl), _fp(gv), _fp(gk)))
+        mask = np.zeros(self.n_worlds, np.uint8)
+        self._chk(self._lib.bg_rollout_nonfinite(self._r, mask.ctypes.data_as(C.POINTER(C.c_uint8))))
+        self.nonfinite_worlds = np.flatnonzero(mask)
         return gv, gk
 
 
 def rollout_function():
     """torch.autograd.Function: (velocity_request [n,2], stiffness [s]) ->
-    final positions [n,2] (float32, CPU tensors; the GPU work happens
-    behind the C ABI)."""
+    final positions [n,2] (float32, on the device of velocity_request; the
+    GPU work happens behind the C ABI, the tensors cross it through host
+    memory).  The rollout keeps one forward's checkpoints: a backward after
+    another forward on the same Rollout raises instead of returning the later
+    call's gradients."""
     import torch
 
     class RolloutFunction(torch.autograd.Function):
@@ -91,11 +103,17 @@ def rollout_function():
             rollout.set_params(vreq.detach().cpu().numpy(), k.detach().cpu().numpy())
             rollout.forward(delta, n_steps)
             ctx.rollout = rollout
-            return torch.from_numpy(rollout.final_positions())
+            ctx.generation = rollout.generation
+            ctx.devices = (vreq.device, k.device)
+            return torch.from_numpy(rollout.final_positions()).to(vreq.device)
 
         @staticmethod
         def backward(ctx, grad_pos):
+            if ctx.rollout.generation != ctx.generation:
+                raise RuntimeError("the Rollout ran another forward after this graph was recorded: its checkpoints "
+                                   "now belong to the later call (use one Rollout per pending backward)")
             gv, gk = ctx.rollout.backward(grad_pos.detach().cpu().contiguous().numpy())
-            return torch.from_numpy(gv), torch.from_numpy(gk), None, None, None
+            return (torch.from_numpy(gv).to(ctx.devices[0]), torch.from_numpy(gk).to(ctx.devices[1]), None, None,
+                    None)
 
     return RolloutFunction

@@ -190,8 +190,15 @@ int bg_remove_col(bg_world *w, bg_handle h);
 int bg_step(bg_world *w, double delta, uint32_t n_steps);
 /* Physics::fixed_step(frame_time), physics.rs:84-99. */
 int bg_fixed_step(bg_world *w, double frame_time);
-/* Wait for queued work on the world's stream. */
+/* Wait for queued work on the world's stream (and report a deferred panic). */
 int bg_synchronize(bg_world *w);
+/* Deferred checks (enable != 0): bg_step returns once its kernels are queued,
+ * without waiting for the GPU; the step's panic flags (physics.rs:293,
+ * 447-448, 471-474) travel back behind the kernels and are reported by a
+ * later bg_step or by bg_synchronize.  Default off: every bg_step waits and
+ * reports its own panic, as the reference's step panics in place.  No
+ * counterpart in the reference (its step runs on the calling thread). */
+int bg_set_deferred_checks(bg_world *w, int enable);
 /* n_chunks x chunk_steps calls of Physics::step(delta) (physics.rs:78-82),
  * each chunk replayed from a hipGraph captured once (launch overhead paid
  * once per capture, not per kernel).  Same results as bg_step with
@@ -320,6 +327,11 @@ int bg_rollout_forward(bg_rollout *r, double delta, uint32_t n_steps, float *los
 int bg_rollout_backward(bg_rollout *r, const float *dloss_dfinal_position, float *grad_velocity_request,
                         float *grad_stiffness);
 int bg_rollout_final_positions(bg_rollout *r, float *out);
+/* After a backward: per_world[w] = 1 if world w's gradients hold inf or NaN
+ * (the fixed-pair derivative of the contact projection grows with the
+ * horizon and can leave the f32 range).  No counterpart in the reference,
+ * which has no adjoint. */
+int bg_rollout_nonfinite(bg_rollout *r, uint8_t *per_world);
 
 /* ---- profiling ------------------------------------------------------- */
 

@@ -10,13 +10,28 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* ---- f32 restatement (the parity oracle) ---- */
-#define REAL float
-#define SFX(x) x##_f32
-#define SQRT sqrtf
+/* ---- f32 restatement (the parity oracle) ----
+ * Rust's f32::sin_cos / f32::atan2 call the platform libm (glibc's sinf,
+ * cosf, atan2f on Linux), whose last bit is not pinned by any standard.
+ * Built with -DORACLE_LIBM_CR (liboracle_cr.so) they are instead evaluated
+ * in double and rounded to f32 -- the correctly rounded result except for
+ * rare double-rounding cases -- which isolates every other operation of the
+ * step from the choice of libm. */
+#ifdef ORACLE_LIBM_CR
+static float cr_sinf(float x) { return (float)sin((double)x); }
+static float cr_cosf(float x) { return (float)cos((double)x); }
+static float cr_atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
+#define SIN cr_sinf
+#define COS cr_cosf
+#define ATAN2 cr_atan2f
+#else
 #define SIN sinf
 #define COS cosf
 #define ATAN2 atan2f
+#endif
+#define REAL float
+#define SFX(x) x##_f32
+#define SQRT sqrtf
 #define ISINF(x) isinf(x)
 #include "oracle_impl.inc"
 #undef REAL

@@ -21,8 +21,11 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
+# libm="cr": the f32 restatement with sinf/cosf/atan2f evaluated in double and
+# rounded (the platform libm's last bit is not part of the reference's semantics)
+LIBS = {"glibc": LIB, "cr": HERE / "liboracle_cr.so"}
 
-_lib = None
+_libs = {}
 
 
 def build(force: bool = False) -> Path:
@@ -31,12 +34,13 @@ def build(force: bool = False) -> Path:
     return LIB
 
 
-def lib():
-    global _lib
+def lib(libm: str = "glibc"):
+    _lib = _libs.get(libm)
     if _lib is None:
-        if not LIB.exists():
+        if not LIBS[libm].exists():
             build()
-        _lib = C.CDLL(str(LIB))
+        _lib = C.CDLL(str(LIBS[libm]))
+        _libs[libm] = _lib
         for sfx, W in (("f32", World32), ("f64", World64)):
             getattr(_lib, f"oracle_step_{sfx}").argtypes = [C.POINTER(W), C.c_double, C.c_int32]
             getattr(_lib, f"oracle_fixed_step_{sfx}").argtypes = [C.POINTER(W), C.c_double,
@@ -116,9 +120,9 @@ def mass_setup(scene):
     return mass, inertia, com.reshape(-1, 2)
 
 
-def joint_setup(scene):
+def joint_setup(scene, libm="glibc"):
     """distance / target_angle of create_fixed_joint for every joint (f32)."""
-    L = lib()
+    L = lib(libm)
     nj = scene.joint_a.shape[0]
     dist = np.zeros(nj, np.float32)
     targ = np.zeros(nj, np.float32)
@@ -144,14 +148,15 @@ class OracleWorld:
     """A scene stepped by the CPU restatement (float32 or float64)."""
 
     def __init__(self, scene, dtype=np.float32, use_grid=True, event_cap=0, substeps=8, joint_iterations=4,
-                 collisions_enabled=True):
+                 collisions_enabled=True, libm="glibc"):
         self.dtype = np.dtype(dtype)
+        self._L = lib(libm)
         R = self.dtype
         self._W = World32 if R == np.float32 else World64
         self._sfx = "f32" if R == np.float32 else "f64"
         n, c = scene.n_bodies, scene.n_colliders
         mass, inertia, _ = mass_setup(scene)
-        jd, jt = joint_setup(scene)
+        jd, jt = joint_setup(scene, libm)
         start, order = scene.body_col_csr()
         a = {}
         a["pos"] = np.ascontiguousarray(scene.pos, R).reshape(-1).copy()
@@ -215,21 +220,21 @@ class OracleWorld:
         self.w = w
 
     def step(self, delta, n=1):
-        getattr(lib(), f"oracle_step_{self._sfx}")(C.byref(self.w), float(delta), int(n))
+        getattr(self._L, f"oracle_step_{self._sfx}")(C.byref(self.w), float(delta), int(n))
 
     def fixed_step(self, frame_time, accumulator: C.c_double):
-        getattr(lib(), f"oracle_fixed_step_{self._sfx}")(C.byref(self.w), float(frame_time), C.byref(accumulator))
+        getattr(self._L, f"oracle_fixed_step_{self._sfx}")(C.byref(self.w), float(frame_time), C.byref(accumulator))
 
     def contact_pairs(self, cap=1 << 22):
         """Run one contact pass (mutating positions) and return its (i, j) pairs."""
         out = np.zeros(2 * cap, np.int32)
-        fn = getattr(lib(), f"oracle_contact_pairs_{self._sfx}")
+        fn = getattr(self._L, f"oracle_contact_pairs_{self._sfx}")
         n = fn(C.byref(self.w), out.ctypes.data_as(C.POINTER(C.c_int32)), cap)
         return out[: 2 * min(n, cap)].reshape(-1, 2)
 
     def bruteforce_rows(self, i0, i1):
         """Run rows [i0, i1) of the reference O(C^2) contact loop; returns pair tests."""
-        return lib().oracle_bruteforce_rows_f32(C.byref(self.w), int(i0), int(i1))
+        return self._L.oracle_bruteforce_rows_f32(C.byref(self.w), int(i0), int(i1))
 
     @property
     def pos(self):
