@@ -204,6 +204,12 @@ SIGNATURES.update({
     "bg_rollout_nonfinite": [ROLLOUT, C.POINTER(C.c_uint8)],
 })
 
+# SpatialHash query (device pointers, passed as integers)
+SIGNATURES.update({
+    "bg_spatial_hash_query": [C.c_float, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                              C.c_uint32, u64p],
+})
+
 # entry points bound only when present in the loaded library
 EXTRA_SIGNATURES: dict[str, list] = {}
 

@@ -333,6 +333,51 @@ int bg_rollout_final_positions(bg_rollout *r, float *out);
  * which has no adjoint. */
 int bg_rollout_nonfinite(bg_rollout *r, uint8_t *per_world);
 
+/* ---- SpatialHash queries (GPU) ---------------------------------------- */
+
+/* One cell membership of SpatialHash (spatial.rs:30-41): a point (id,
+ * position, radius, spatial.rs:10-15) filed under `cell`.  Normally one per
+ * point with cell = get_cell_coords(position) (spatial.rs:55-60); after
+ * insert_with_id of an existing id the old cell keeps its entry
+ * (spatial.rs:69-87), i.e. a second record of the id under the old cell. */
+typedef struct {
+    uint64_t id;
+    float position[2];
+    float radius;
+    int32_t cell[2];
+    uint32_t pad;
+} bg_spatial_point;
+
+/* One SpatialHash::query(position, query_radius) (spatial.rs:155-195). */
+typedef struct {
+    float position[2];
+    float radius;
+    uint32_t pad;
+} bg_spatial_query;
+
+/* One CellPoint of a query's result; `point` is the record's index. */
+typedef struct {
+    uint64_t id;
+    float position[2];
+    float radius;
+    uint32_t point;
+} bg_spatial_hit;
+
+/* SpatialHash::query (spatial.rs:155-195) for n_queries queries over
+ * n_points memberships, SpatialHash::new(cell_size) (spatial.rs:43-53).
+ * All arrays are device memory.  offsets (n_queries + 1 entries) receives
+ * the exclusive scan of the per-query hit counts; the hits of query q are
+ * hits[offsets[q] .. offsets[q+1]): points of the 3x3 cells around the
+ * query's cell (get_neighbor_cells order, spatial.rs:140-153) with
+ * |p - q|^2 <= (query_radius + p.radius)^2, in record order inside a cell
+ * (the reference's HashSet order is unspecified).  *total_hits = offsets[n].
+ * Hits past hit_cap are not written and the call returns BG_ERR_CAPACITY
+ * (offsets stay complete).  Scratch of about 40 bytes per point is
+ * allocated and freed by the call; runs on the null stream and waits. */
+int bg_spatial_hash_query(float cell_size, const bg_spatial_point *points, uint32_t n_points,
+                     const bg_spatial_query *queries, uint32_t n_queries, uint32_t *offsets, bg_spatial_hit *hits,
+                     uint32_t hit_cap, uint64_t *total_hits);
+
 /* ---- profiling ------------------------------------------------------- */
 
 /* Per-stage timing with HIP events on the world's stream, the analogue of

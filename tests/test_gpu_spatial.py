@@ -1,0 +1,104 @@
+"""SpatialHash queries on the GPU (bg_spatial_hash_query) against the
+restatement of spatial.rs (oracle/spatial_oracle.py, pinned by the
+reference's tests.rs cases in test_spatial_oracle.py).  Bar: every query's
+hits identical to the oracle's -- same ids, positions and radii, in the same
+order (neighbour-cell order, insertion order inside a cell) -- and equal to
+a brute-force radius search wherever query radius + largest point radius
+fits in one cell."""
+import numpy as np
+import pytest
+
+from blobs_amd import _lib as L
+from blobs_amd.spatial import QUERY, SpatialHash, query_points
+from oracle.spatial_oracle import SpatialHashOracle
+
+pytestmark = pytest.mark.gpu
+
+
+def build(cell, pts, rad, ids=None):
+    h, o = SpatialHash(cell), SpatialHashOracle(cell)
+    for k, (p, r) in enumerate(zip(pts, rad)):
+        if ids is None:
+            h.insert(p, r)
+            o.insert(p, r)
+        else:
+            h.insert_with_id(ids[k], p, r)
+            o.insert_with_id(ids[k], p, r)
+    return h, o
+
+
+def assert_same(got, want, what):
+    g = [(int(x["id"]), float(x["position"][0]), float(x["position"][1]), float(x["radius"])) for x in got]
+    w = [(p[0], float(p[1][0]), float(p[1][1]), float(p[2])) for p in want]
+    assert g == w, f"{what}: gpu {g[:4]}... vs oracle {w[:4]}... ({len(g)} vs {len(w)})"
+
+
+def test_reference_cases_on_the_gpu():
+    """tests.rs:54-89 through the GPU query."""
+    h = SpatialHash(1.0)
+    p1, p2 = h.insert((0.0, 0.0), 0.0), h.insert((2.0, 2.0), 0.0)
+    assert sorted(int(x) for x in h.query((1.0, 1.0), 1.5)["id"]) == [p1, p2]
+    h = SpatialHash(1.0)
+    p = h.insert((0.0, 0.0), 0.0)
+    h.move_point(p, (2.0, 2.0))
+    r = h.query((1.0, 1.0), 1.5)
+    assert r.shape[0] == 1 and int(r["id"][0]) == p and tuple(r["position"][0]) == (2.0, 2.0)
+    h = SpatialHash(1.0)
+    h.remove(h.insert((0.0, 0.0), 0.0))
+    assert h.query((0.0, 0.0), 1.5).shape[0] == 0
+
+
+@pytest.mark.parametrize("n,cell", [(1, 2.0), (5000, 2.0), (200000, 1.0)])
+def test_random_points_match_the_oracle_and_brute_force(n, cell):
+    rng = np.random.default_rng(n)
+    side = np.sqrt(n) * 1.5
+    pts = rng.uniform(-side, side, (n, 2)).astype(np.float32)
+    rad = rng.uniform(0.0, 0.25 * cell, n).astype(np.float32)
+    h, o = build(cell, pts, rad)
+    nq = min(4000, 4 * n)
+    qs = rng.uniform(-side - cell, side + cell, (nq, 2)).astype(np.float32)
+    qr = rng.uniform(0.0, 0.7 * cell, nq).astype(np.float32)
+    got = h.query_many(qs, qr)
+    for k in range(nq):
+        want = o.query(qs[k], qr[k])
+        assert_same(got[k], want, f"query {k}")
+        if k < 300:  # radius + max point radius < cell: the 3x3 cells see everything in range
+            d2 = ((pts - qs[k]) ** 2).sum(1, dtype=np.float32)
+            brute = np.flatnonzero(d2 <= (qr[k] + rad) ** 2)
+            assert sorted(int(i) for i in got[k]["id"]) == sorted(brute.tolist())
+
+
+def test_edge_cases_match_the_oracle():
+    """Cell boundaries, negative and huge coordinates (the i32 cast saturates,
+    neighbour cells wrap), NaN (cell 0), zero radii, a re-inserted id (its old
+    cell keeps an entry), a removed point, and queries far from everything."""
+    pts = [(0.0, 0.0), (-0.0, 2.0), (2.0, -2.0), (1.9999999, 0.0), (-1e-7, -1e-7), (3e9, 3e9), (-3e9, 3e9),
+           (np.nan, 1.0), (1.0, np.nan), (np.inf, 0.0), (4.0, 4.0), (1e10, 1e10), (-1e10, 5.0)]
+    rad = [0.0, 0.5, 1.0, 0.1, 0.0, 1e10, 1e10, 1.0, 1.0, 1.0, 0.3, 1e11, 1e11]
+    h, o = build(2.0, pts, rad)
+    for hh in (h, o):
+        hh.insert_with_id(10, (2.5, 2.5), 0.3)  # id 10 again: its old cell (2, 2) keeps an entry
+        hh.remove(2)
+        hh.move_point(3, (0.0000001, 0.0))
+    qs = [(0.0, 0.0), (1.0, 1.0), (-1.0, -1.0), (3e9, 3e9), (2.5e9, 3e9), (np.nan, np.nan), (1.0, 0.0),
+          (4.0, 4.0), (1e6, -1e6), (-2.0, 1.9), (9e9, 1e10), (-1e10, 4.0), (3.0, 3.0)]
+    qr = [3.0, 0.0, 1.0, 1.0, 1e10, 5.0, 0.5, 10.0, 1.0, 2.0, 1e10, 1.0, 1.0]
+    got = h.query_many(qs, qr)
+    for k in range(len(qs)):
+        assert_same(got[k], o.query(qs[k], qr[k]), f"edge query {k} at {qs[k]}")
+    assert [int(x) for x in got[-1]["id"]].count(10) == 2
+
+
+def test_empty_inputs_and_capacity():
+    off, hits = query_points(2.0, SpatialHash(2.0).memberships(), np.zeros(3, QUERY))
+    assert off.tolist() == [0, 0, 0, 0] and hits.shape[0] == 0
+    h = SpatialHash(1.0)
+    for k in range(50):
+        h.insert((0.1 * (k % 5), 0.1 * (k // 5)), 0.0)
+    q = np.zeros(4, QUERY)
+    q["radius"] = 5.0
+    with pytest.raises(L.BlobsError) as e:
+        query_points(1.0, h.memberships(), q, hit_cap=100)
+    assert e.value.code == L.BG_ERR_CAPACITY
+    off, hits = query_points(1.0, h.memberships(), q, hit_cap=200)
+    assert off.tolist() == [0, 50, 100, 150, 200] and hits.shape[0] == 200
