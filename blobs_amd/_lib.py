@@ -208,10 +208,19 @@ SIGNATURES.update({
 SIGNATURES.update({
     "bg_spatial_hash_query": [C.c_float, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                               C.c_uint32, u64p],
+    "bg_spatial_hash_query_host": [C.c_float, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                   C.c_void_p, C.c_uint32, u64p],
 })
-
 # entry points bound only when present in the loaded library
 EXTRA_SIGNATURES: dict[str, list] = {}
+# device-buffer test hooks (internal)
+EXTRA_SIGNATURES.update({
+    "bgx_device_alloc": [C.c_uint64, C.POINTER(C.c_void_p)],
+    "bgx_device_free": [C.c_void_p],
+    "bgx_copy_to_device": [C.c_void_p, C.c_void_p, C.c_uint64],
+    "bgx_copy_to_host": [C.c_void_p, C.c_void_p, C.c_uint64],
+})
+
 
 _lib: C.CDLL | None = None
 

@@ -24,6 +24,15 @@ namespace {
 
 thread_local std::string g_last_error;
 
+}  // namespace
+
+namespace bg {
+// Message of a failed call that has no world (read by bg_last_error(NULL, ...)).
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace bg
+
+namespace {
+
 template <class F>
 int guarded(bg_world* w, F&& f) {
     if (!w) {

@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 
 #include "../../include/blobs_gpu.h"
 #include "common.hpp"
@@ -32,7 +33,15 @@
 
 namespace bg {
 
+void set_last_error(const std::string& msg);
+
 namespace {
+
+int fail(int code, const std::string& msg) {
+    set_last_error(msg);
+    return code;
+}
+
 
 // Key of cell (cx, cy); the one cell whose key equals kEmpty is filed in the
 // extra slot T of the table instead.
@@ -190,20 +199,21 @@ __global__ void k_sh_fill(const bg_spatial_point* __restrict__ pts, const bg_spa
     });
 }
 
-uint32_t table_size(uint32_t n) {
+constexpr uint32_t table_size(uint32_t n) {
     uint32_t t = 64;
     while (t < 2u * n) t <<= 1;
     return t;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Bytes of scratch for n records and nq queries, and the views into it.
-size_t scratch_bytes(uint32_t n, uint32_t nq) {
+constexpr size_t scratch_bytes(uint32_t n, uint32_t nq) {
     const uint32_t T = table_size(n);
-    const uint32_t tiles = div_up(std::max(T + 2, nq + 1), kScanTile);
+    const uint32_t m = T + 2 > nq + 1 ? T + 2 : nq + 1;
+    const uint32_t tiles = (m + kScanTile - 1) / kScanTile;
     return align256(sizeof(unsigned long long) * T) + align256(sizeof(uint32_t) * (T + 2)) +
-           3 * align256(sizeof(uint32_t) * std::max(n, 1u)) + align256(sizeof(uint32_t) * tiles);
+           3 * align256(sizeof(uint32_t) * (n > 1u ? n : 1u)) + align256(sizeof(uint32_t) * tiles);
 }
 
 SpatialScratch carve(char* base, uint32_t n, uint32_t nq) {
@@ -251,18 +261,20 @@ hipError_t spatial_query_on(const SpatialScratch& s, float cs, const bg_spatial_
 
 int check_args(float cs, const bg_spatial_point* pts, uint32_t n, const bg_spatial_query* qs, uint32_t nq,
                const uint32_t* offsets, const bg_spatial_hit* hits, uint32_t cap) {
-    if (!offsets || (n > 0 && !pts) || (nq > 0 && !qs) || (cap > 0 && !hits)) return BG_ERR_INVALID_ARGUMENT;
-    if (n > (1u << 30) || nq > (1u << 30)) return BG_ERR_INVALID_ARGUMENT;
+    if (!offsets || (n > 0 && !pts) || (nq > 0 && !qs) || (cap > 0 && !hits))
+        return fail(BG_ERR_INVALID_ARGUMENT, "spatial query: offsets, and points / queries / hits when counted, are required");
+    if (n > (1u << 30) || nq > (1u << 30)) return fail(BG_ERR_INVALID_ARGUMENT, "spatial query: more than 2^30 records");
     (void)cs;
     return BG_OK;
 }
 
 int finish(hipError_t e, const uint32_t* offsets, uint32_t nq, uint32_t cap, uint64_t* total_out) {
-    if (e != hipSuccess) return BG_ERR_HIP;
+    if (e != hipSuccess) return fail(BG_ERR_HIP, std::string("spatial query: ") + hipGetErrorString(e));
     uint32_t total = 0;
-    if (hipMemcpy(&total, offsets + nq, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return BG_ERR_HIP;
+    e = hipMemcpy(&total, offsets + nq, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(BG_ERR_HIP, std::string("spatial query: total: ") + hipGetErrorString(e));
     if (total_out) *total_out = total;
-    return total > cap ? BG_ERR_CAPACITY : BG_OK;
+    return total > cap ? fail(BG_ERR_CAPACITY, "spatial query: more hits than hit_cap (offsets are complete)") : BG_OK;
 }
 
 }  // namespace
@@ -277,7 +289,8 @@ int bg_spatial_hash_query(float cell_size, const bg_spatial_point* points, uint3
     using namespace bg;
     if (int st = check_args(cell_size, points, n_points, queries, n_queries, offsets, hits, hit_cap)) return st;
     char* mem = nullptr;
-    if (hipMalloc((void**)&mem, scratch_bytes(n_points, n_queries)) != hipSuccess) return BG_ERR_HIP;
+    const hipError_t ea = hipMalloc((void**)&mem, scratch_bytes(n_points, n_queries));
+    if (ea != hipSuccess) return fail(BG_ERR_HIP, std::string("spatial query: scratch: ") + hipGetErrorString(ea));
     const SpatialScratch s = carve(mem, n_points, n_queries);
     hipError_t e = spatial_query_on(s, cell_size, points, n_points, queries, n_queries, offsets, hits, hit_cap,
                                     nullptr);
@@ -285,6 +298,53 @@ int bg_spatial_hash_query(float cell_size, const bg_spatial_point* points, uint3
     const int st = finish(e, offsets, n_queries, hit_cap, total_hits);
     (void)hipFree(mem);
     return st;
+}
+
+int bg_spatial_hash_query_host(float cell_size, const bg_spatial_point* points, uint32_t n_points,
+                               const bg_spatial_query* queries, uint32_t n_queries, uint32_t* offsets,
+                               bg_spatial_hit* hits, uint32_t hit_cap, uint64_t* total_hits) {
+    using namespace bg;
+    if (int st = check_args(cell_size, points, n_points, queries, n_queries, offsets, hits, hit_cap)) return st;
+    const size_t bp = sizeof(bg_spatial_point) * std::max(n_points, 1u);
+    const size_t bq = sizeof(bg_spatial_query) * std::max(n_queries, 1u);
+    const size_t bo = sizeof(uint32_t) * ((size_t)n_queries + 1);
+    const size_t bh = sizeof(bg_spatial_hit) * std::max(hit_cap, 1u);
+    char* dev = nullptr;
+    hipError_t e = hipMalloc((void**)&dev, bp + bq + bo + bh + 4 * 256);
+    if (e != hipSuccess) return fail(BG_ERR_HIP, std::string("spatial query: buffers: ") + hipGetErrorString(e));
+    bg_spatial_point* dp = (bg_spatial_point*)dev;
+    bg_spatial_query* dq = (bg_spatial_query*)(dev + align256(bp));
+    uint32_t* doff = (uint32_t*)((char*)dq + align256(bq));
+    bg_spatial_hit* dh = (bg_spatial_hit*)((char*)doff + align256(bo));
+    int st = BG_OK;
+    if ((n_points && (e = hipMemcpy(dp, points, sizeof(bg_spatial_point) * n_points, hipMemcpyHostToDevice))) ||
+        (n_queries && (e = hipMemcpy(dq, queries, sizeof(bg_spatial_query) * n_queries, hipMemcpyHostToDevice)))) {
+        st = fail(BG_ERR_HIP, std::string("spatial query: upload: ") + hipGetErrorString(e));
+    } else {
+        uint64_t total = 0;
+        st = bg_spatial_hash_query(cell_size, dp, n_points, dq, n_queries, doff, dh, hit_cap, &total);
+        if (st == BG_OK || st == BG_ERR_CAPACITY) {
+            const size_t nh = std::min<uint64_t>(total, hit_cap);
+            if ((e = hipMemcpy(offsets, doff, bo, hipMemcpyDeviceToHost)) ||
+                (nh && (e = hipMemcpy(hits, dh, sizeof(bg_spatial_hit) * nh, hipMemcpyDeviceToHost))))
+                st = fail(BG_ERR_HIP, std::string("spatial query: download: ") + hipGetErrorString(e));
+            if (total_hits) *total_hits = total;
+        }
+    }
+    (void)hipFree(dev);
+    return st;
+}
+
+// ---- device-buffer test hooks (bgx_: internal, not part of the Physics ABI) ----
+int bgx_device_alloc(uint64_t bytes, void** out) {
+    return hipMalloc(out, bytes ? bytes : 1) == hipSuccess ? BG_OK : BG_ERR_HIP;
+}
+int bgx_device_free(void* p) { return hipFree(p) == hipSuccess ? BG_OK : BG_ERR_HIP; }
+int bgx_copy_to_device(void* dst, const void* src, uint64_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? BG_OK : BG_ERR_HIP;
+}
+int bgx_copy_to_host(void* dst, const void* src, uint64_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? BG_OK : BG_ERR_HIP;
 }
 
 }  // extern "C"

@@ -4,7 +4,7 @@
 points by id, and per cell (floor(x / cell_size), floor(y / cell_size)) as
 i32 the ids filed there (spatial.rs:30-138), including the entry an
 insert_with_id of an existing id leaves in the old cell -- and answers
-`query` / `query_many` with bg_spatial_hash_query: the memberships and
+`query` / `query_many` with bg_spatial_hash_query_host: the memberships and
 queries go to device memory, one launch sequence hashes the occupied cells,
 files the memberships into per-cell runs and evaluates every query
 (spatial.rs:155-195).  A query's hits come in the reference's neighbour-cell
@@ -37,37 +37,27 @@ def _cast_i32(f: float) -> int:
     return int(f)
 
 
-def _device(a: np.ndarray):
-    import torch
-    t = torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1).copy())
-    return t.to("cuda")
-
-
 def query_points(cell_size: float, points: np.ndarray, queries: np.ndarray, hit_cap: int | None = None):
     """The GPU query over explicit memberships (POINT records) for QUERY
-    records: (offsets [n_queries + 1], hits [offsets[-1]] as HIT records)."""
-    import torch
+    records, from host arrays (bg_spatial_hash_query_host): (offsets
+    [n_queries + 1], hits [offsets[-1]] as HIT records)."""
     lib = L.load()
     points = np.ascontiguousarray(points, POINT)
     queries = np.ascontiguousarray(queries, QUERY)
-    dp = _device(points) if points.size else torch.zeros(32, dtype=torch.uint8, device="cuda")
-    dq = _device(queries) if queries.size else torch.zeros(16, dtype=torch.uint8, device="cuda")
-    offsets = torch.zeros(queries.shape[0] + 1, dtype=torch.int32, device="cuda")
+    offsets = np.zeros(queries.shape[0] + 1, np.uint32)
     cap = 1 << 16 if hit_cap is None else int(hit_cap)
     total = C.c_uint64()
     for _ in range(2):
-        hits = torch.zeros(max(cap, 1) * HIT.itemsize, dtype=torch.uint8, device="cuda")
-        code = lib.bg_spatial_hash_query(float(cell_size), dp.data_ptr(), points.shape[0], dq.data_ptr(),
-                                         queries.shape[0], offsets.data_ptr(), hits.data_ptr(), cap, C.byref(total))
+        hits = np.zeros(max(cap, 1), HIT)
+        code = lib.bg_spatial_hash_query_host(float(cell_size), points.ctypes.data, points.shape[0],
+                                              queries.ctypes.data, queries.shape[0], offsets.ctypes.data,
+                                              hits.ctypes.data, cap, C.byref(total))
         if code == L.BG_ERR_CAPACITY and hit_cap is None:
             cap = int(total.value)  # offsets are complete: size the buffer once more
             continue
         L.check(code, None)
         break
-    off = offsets.cpu().numpy().view(np.uint32).astype(np.int64)
-    n = int(total.value)
-    out = hits.cpu().numpy()[: n * HIT.itemsize].view(HIT) if n else np.zeros(0, HIT)
-    return off, out
+    return offsets.astype(np.int64), hits[: int(total.value)]
 
 
 class SpatialHash:

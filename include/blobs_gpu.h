@@ -378,6 +378,13 @@ int bg_spatial_hash_query(float cell_size, const bg_spatial_point *points, uint3
                      const bg_spatial_query *queries, uint32_t n_queries, uint32_t *offsets, bg_spatial_hit *hits,
                      uint32_t hit_cap, uint64_t *total_hits);
 
+/* bg_spatial_hash_query on host arrays: copies them to the device, runs the
+ * query, copies offsets and hits back (for callers whose SpatialHash lives
+ * in host memory, as the reference's does). */
+int bg_spatial_hash_query_host(float cell_size, const bg_spatial_point *points, uint32_t n_points,
+                               const bg_spatial_query *queries, uint32_t n_queries, uint32_t *offsets,
+                               bg_spatial_hit *hits, uint32_t hit_cap, uint64_t *total_hits);
+
 /* ---- profiling ------------------------------------------------------- */
 
 /* Per-stage timing with HIP events on the world's stream, the analogue of

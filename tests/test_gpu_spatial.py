@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from blobs_amd import _lib as L
-from blobs_amd.spatial import QUERY, SpatialHash, query_points
+from blobs_amd.spatial import HIT, QUERY, SpatialHash, query_points
+from tests.devbuf import DevBuf
 from oracle.spatial_oracle import SpatialHashOracle
 
 pytestmark = pytest.mark.gpu
@@ -102,3 +103,28 @@ def test_empty_inputs_and_capacity():
     assert e.value.code == L.BG_ERR_CAPACITY
     off, hits = query_points(1.0, h.memberships(), q, hit_cap=200)
     assert off.tolist() == [0, 50, 100, 150, 200] and hits.shape[0] == 200
+
+
+def test_device_pointer_entry_point_equals_the_host_one():
+    """bg_spatial_hash_query on device buffers gives the bytes the host-buffer
+    entry point (which wraps it) returns."""
+    import ctypes as C
+    rng = np.random.default_rng(9)
+    h = SpatialHash(1.5)
+    for p, r in zip(rng.uniform(-80, 80, (20000, 2)).astype(np.float32), rng.uniform(0, 0.4, 20000)):
+        h.insert(p, r)
+    pts = h.memberships()
+    q = np.zeros(5000, QUERY)
+    q["position"] = rng.uniform(-82, 82, (5000, 2))
+    q["radius"] = rng.uniform(0, 1.0, 5000)
+    off_h, hits_h = query_points(1.5, pts, q)
+    lib = L.load()
+    dp, dq = DevBuf(lib, pts.nbytes, pts), DevBuf(lib, q.nbytes, q)
+    cap = int(off_h[-1])
+    doff, dh = DevBuf(lib, 4 * (q.shape[0] + 1)), DevBuf(lib, cap * HIT.itemsize)
+    total = C.c_uint64()
+    L.check(lib.bg_spatial_hash_query(1.5, dp.p, pts.shape[0], dq.p, q.shape[0], doff.p, dh.p, cap,
+                                      C.byref(total)), None)
+    assert total.value == cap > 0
+    assert np.array_equal(doff.get(np.uint32, q.shape[0] + 1).astype(np.int64), off_h)
+    assert np.array_equal(dh.get(HIT, cap).view(np.uint8), hits_h.view(np.uint8))
