@@ -208,6 +208,8 @@ SIGNATURES.update({
 SIGNATURES.update({
     "bg_spatial_hash_query": [C.c_float, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                               C.c_uint32, u64p],
+    "bg_spatial_hash_query_noalloc": [C.c_float, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.c_void_p, C.c_uint32, u64p],
     "bg_spatial_hash_query_host": [C.c_float, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                    C.c_void_p, C.c_uint32, u64p],
 })

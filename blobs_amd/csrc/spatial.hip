@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/blobs_gpu.h"
@@ -216,6 +217,35 @@ constexpr size_t scratch_bytes(uint32_t n, uint32_t nq) {
            3 * align256(sizeof(uint32_t) * (n > 1u ? n : 1u)) + align256(sizeof(uint32_t) * tiles);
 }
 
+// The no-allocation entry point's scratch: one array per GPU, held until the
+// process ends, large enough for kNoAllocMaxPoints records and
+// kNoAllocMaxQueries queries (36 MiB).  Host code never passes `scratch`
+// itself to a kernel (in host code the name is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress returns per device.
+constexpr uint32_t kNoAllocMaxPoints = 1u << 20, kNoAllocMaxQueries = 1u << 20;
+constexpr size_t kScratchBytes = scratch_bytes(kNoAllocMaxPoints, kNoAllocMaxQueries);
+static_assert(kScratchBytes == 37751296, "36 MiB of scratch");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_on_current_device(char** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return BG_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return fail(BG_ERR_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = (char*)g_scratch_ptr[dev];
+    return BG_OK;
+}
+
 SpatialScratch carve(char* base, uint32_t n, uint32_t nq) {
     SpatialScratch s;
     s.T = table_size(n);
@@ -345,6 +375,23 @@ int bgx_copy_to_device(void* dst, const void* src, uint64_t bytes) {
 }
 int bgx_copy_to_host(void* dst, const void* src, uint64_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? BG_OK : BG_ERR_HIP;
+}
+
+int bg_spatial_hash_query_noalloc(float cell_size, const bg_spatial_point* points, uint32_t n_points,
+                                  const bg_spatial_query* queries, uint32_t n_queries, uint32_t* offsets,
+                                  bg_spatial_hit* hits, uint32_t hit_cap, uint64_t* total_hits) {
+    using namespace bg;
+    if (int st = check_args(cell_size, points, n_points, queries, n_queries, offsets, hits, hit_cap)) return st;
+    if (n_points > kNoAllocMaxPoints || n_queries > kNoAllocMaxQueries)
+        return fail(BG_ERR_CAPACITY, "spatial query without allocation: at most 1,048,576 points and 1,048,576 queries");
+    char* mem = nullptr;
+    if (int st = scratch_on_current_device(&mem)) return st;
+    // every table word the call reads is cleared by its first kernel (k_sh_clear)
+    const SpatialScratch s = carve(mem, n_points, n_queries);
+    hipError_t e = spatial_query_on(s, cell_size, points, n_points, queries, n_queries, offsets, hits, hit_cap,
+                                    nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return finish(e, offsets, n_queries, hit_cap, total_hits);
 }
 
 }  // extern "C"

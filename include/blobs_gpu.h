@@ -385,6 +385,17 @@ int bg_spatial_hash_query_host(float cell_size, const bg_spatial_point *points, 
                                const bg_spatial_query *queries, uint32_t n_queries, uint32_t *offsets,
                                bg_spatial_hit *hits, uint32_t hit_cap, uint64_t *total_hits);
 
+/* bg_spatial_hash_query without any allocation: the same arguments, results
+ * and kernels, with the scratch in a device array the library holds on each
+ * GPU until the process ends (36 MiB).  For callers that cannot allocate
+ * device memory (code under stream capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Limits: at most
+ * 1,048,576 points and 1,048,576 queries (BG_ERR_CAPACITY beyond, nothing
+ * launched).  Calls that overlap on one GPU share that array and race. */
+int bg_spatial_hash_query_noalloc(float cell_size, const bg_spatial_point *points, uint32_t n_points,
+                                  const bg_spatial_query *queries, uint32_t n_queries, uint32_t *offsets,
+                                  bg_spatial_hit *hits, uint32_t hit_cap, uint64_t *total_hits);
+
 /* ---- profiling ------------------------------------------------------- */
 
 /* Per-stage timing with HIP events on the world's stream, the analogue of
