@@ -138,7 +138,10 @@ def test_multi_collider_300_steps_bitexact_digests_and_fp64_statistics(multi_col
     f = np.array(ref["fp64"])[win].mean(0)
     o32 = np.array(ref["f32"])[win].mean(0)
     rel = np.abs(g - f) / np.abs(f)
-    bars = [0.35, 0.10, 0.05, 0.15, 0.02]
+    # the oracle's own f32 run differs from FP64 by 0.2%, 0.8%, 1.8%, 0.2% and 8.7% on these
+    # statistics: the f32 step itself falls slower at these coordinates (|y| up to 390, where
+    # p + (disp + g h^2) rounds away part of g h^2 = 4.3e-5 against an ulp of 3e-5)
+    bars = [0.15, 0.05, 0.08, 0.05, 0.15]
     for name, r, bar, gv, fv, ov in zip(NAMES, rel, bars, g, f, o32):
         assert r <= bar, (f"{name}: GPU {gv:.6g} vs FP64 {fv:.6g} ({r:.1%} > {bar:.0%}); "
                           f"the oracle's own f32 run gives {ov:.6g}")
