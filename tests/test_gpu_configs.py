@@ -178,3 +178,35 @@ def test_differentiable_rollout_full_shape_forward_bitexact_on_sampled_worlds():
         assert_bitexact(pos[2048 * w:2048 * (w + 1)], o.pos, f"rollout world {w}")
         cen = o.pos.astype(np.float64).mean(0)
         assert loss[w] == pytest.approx(float(((cen - targets[w]) ** 2).sum()), rel=1e-6)
+
+
+def test_differentiable_rollout_full_shape_gradients_on_sampled_worlds():
+    """Gradients of the full rollout (256 worlds, checkpoint every 20) after 50
+    and 200 steps against the FP64 reverse pass along the same f32 trajectory
+    (fixture: tests/golden/rollout_grads.npz).  Worlds 0 and 1: within 1e-3
+    relative (vector norm) for velocity requests and stiffnesses.  Worlds 3
+    and 97: their FP64 gradients exceed the f32 range at 200 steps (the
+    fixed-pair derivative of the contact projection grows about 1e8-fold every
+    50 steps), so the rollout reports them in nonfinite_worlds."""
+    from blobs_amd.rollout import Rollout
+    ref = np.load(GOLDEN / "rollout_grads.npz")
+    ws, targets = scenes.rollout_scenes()
+    r = Rollout([w.build_physics()[0] for w in ws], targets, checkpoint_every=20)
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    for horizon in (50, 200):
+        r.forward(ws[0].dt, horizon)
+        gv, gk = r.backward()
+        for w in (0, 1):
+            ev = rel(gv[2048 * w:2048 * (w + 1)], ref[f"w{w}_h{horizon}_gv"])
+            ek = rel(gk[1024 * w:1024 * (w + 1)], ref[f"w{w}_h{horizon}_gk"])
+            assert ev <= 1e-3 and ek <= 1e-3, f"world {w}, {horizon} steps: rel error {ev:.2e} (v), {ek:.2e} (k)"
+        bad = set(r.nonfinite_worlds.tolist())
+        assert not bad & {0, 1}
+        if horizon == 200:
+            for w in (3, 97):
+                assert float(ref[f"w{w}_h200_maxabs"]) > 3.5e38
+                assert w in bad, f"world {w}: FP64 gradient {float(ref[f'w{w}_h200_maxabs']):.3g} but finite in f32"
