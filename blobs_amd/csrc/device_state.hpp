@@ -145,6 +145,17 @@ struct DeviceWorld {
     int32_t *j_a = nullptr, *j_b = nullptr;
     float2 *j_anchor_a = nullptr, *j_anchor_b = nullptr;
     float *j_dist = nullptr, *j_target = nullptr;
+    // per joint: 1/m_a + 1/m_b, (1/m_a)/ims, 1 - that, and 1 if both masses are positive
+    // (physics.rs:447-460; the same f32 operations, evaluated at upload)
+    float4 *j_mass = nullptr;
+    // deferred joint rotations: per solve (component-major, then iteration, then joint),
+    // the axis record and its rotation correction; per body, its joints in arena order
+    // (entry = position in jc_list << 1 | 1 if the body is the joint's b) and component
+    uint32_t j_iters = 0;  // joint_iterations the records are sized for
+    float4 *jrec = nullptr;
+    float2 *jcorr = nullptr;
+    uint32_t *bj_start = nullptr, *bj_list = nullptr;
+    int32_t *bj_comp = nullptr;
     uint32_t *jc_start = nullptr, *jc_list = nullptr;
     uint32_t n_jcomp = 0;
     // components of at most kJointBodies bodies and kJointJoints joints iterate in LDS:
@@ -160,6 +171,12 @@ struct DeviceWorld {
     uint32_t *cell_start = nullptr;
     float4 *crec = nullptr;  // general path: (x, y, radius, parent bits) of each collider, in cell order
     GridParams *grid = nullptr;
+    // contacts in two kernels (general path): per collider, its kColList smallest-key
+    // corrections [kColList][nc] as (key hi, key lo, cx, cy) and its contact count; the
+    // bodies the per-body merge leaves to the full walk (count in cfb_n[0])
+    float4 *clist = nullptr;
+    uint32_t *ccnt = nullptr;
+    uint32_t *cfb_list = nullptr, *cfb_n = nullptr;
     // ordered contact pass (general path; see OrderedPair)
     uint32_t *ord_rows = nullptr;            // [nc] rows of coincident pairs, count in ord_n[0]
     uint32_t *ord_n = nullptr;               // [1]
@@ -213,6 +230,8 @@ struct SubstepArgs {
     uint32_t sub_global;  // substeps since the world was created (orders collision events)
     uint32_t write_vel;   // sphere path: store calculated_velocity (last substep of a step, or
                           // every substep when collision events read it)
+    uint32_t joint_rot;   // general path: k_integrate applies the joint stage's rotation corrections
+    uint32_t joint_iters;
 };
 
 // Launchers (step_kernels.hip: general path + broadphase build).

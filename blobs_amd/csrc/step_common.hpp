@@ -226,6 +226,38 @@ __device__ __forceinline__ void verlet_dynamic(BodyState& s, float2 acc, float t
     s.vel.y = dy / h;
 }
 
+// Position part of one fixed joint (physics.rs:434-460) on body positions pa,
+// pb; the axis d = wb - wa goes to (dx, dy).  Returns false for coincident
+// anchors, which the reference skips.
+__device__ __forceinline__ bool joint_positions(float2& pa, float2& pb, float2 an_a, float2 an_b, float jdist,
+                                                float4 jm, bool sa, bool sb, float& dx, float& dy,
+                                                uint32_t& flags) {
+    float wax = pa.x + an_a.x, way = pa.y + an_a.y;
+    float wbx = pb.x + an_b.x, wby = pb.y + an_b.y;
+    dx = wbx - wax;
+    dy = wby - way;
+    float dist = vlen(dx, dy);
+    if (dist < 1e-6f) return false;
+    float off_by = dist - jdist;
+    float cx = (off_by * dx) / dist, cy = (off_by * dy) / dist;
+    if (jm.w == 0.0f) flags |= ERR_JOINT_MASS;
+    const float ims = jm.x;
+    if (sa) {
+        pb.x = pb.x - ims * cx;
+        pb.y = pb.y - ims * cy;
+    } else if (sb) {
+        pa.x = pa.x + ims * cx;
+        pa.y = pa.y + ims * cy;
+    } else {
+        const float ratio = jm.y, rbm = jm.z;
+        pa.x = pa.x + ratio * cx;
+        pa.y = pa.y + ratio * cy;
+        pb.x = pb.x - rbm * cx;
+        pb.y = pb.y - rbm * cy;
+    }
+    return true;
+}
+
 // Circle constraints, physics.rs:377-395 (centre clamp, static bodies too).
 __device__ __forceinline__ void apply_constraints(float2& p, const float2* __restrict__ con_pos,
                                                   const float* __restrict__ con_r, uint32_t nk) {

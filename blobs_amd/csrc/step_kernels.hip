@@ -126,17 +126,21 @@ __global__ __launch_bounds__(kThreads) void k_forces(DeviceWorld d, SubstepArgs 
 constexpr uint32_t kColStage = 8;  // colliders per body staged in LDS by k_contacts
 
 // Contacts (physics.rs:241-317), one thread per body over its colliders, in
-// blocks of kContactThreads.
+// blocks of kContactThreads.  With `list`, only the bodies listed there (count
+// in cfb_n[0], grid-stride) and without counting or events: the bodies whose
+// collider lists overflowed in k_contact_lists, which counted their contacts.
 template <bool GROUPS, bool SENSORS>
-__global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, SubstepArgs a) {
+__global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, SubstepArgs a, const uint32_t* list) {
     // the body's first kColStage colliders (x, y, radius, index), loaded together up front
     // so that the walk over its colliders does not wait on two dependent loads per collider
     __shared__ float4 s_col[kColStage][kContactThreads];
     LocalAccum la;
     la.init();
     const uint32_t lane = threadIdx.x;
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < d.nb) {
+    const uint32_t n_items = list ? d.cfb_n[0] : d.nb;
+    const uint32_t stride = list ? gridDim.x * blockDim.x : n_items;  // without a list: one body per thread
+    for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += stride) {
+        const uint32_t b = list ? list[item] : item;
         const GridParams g = *d.grid;
         const float D = read_dmax(d.accum);
         float2 p = d.b.pos[b];
@@ -278,7 +282,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
         };
         enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist, float rs,
                       bool sen_a, float mp) {
-            if (isA) ++count;
+            if (isA && !list) ++count;
             correction(true, [&](uint64_t k, float x, float y) { L.add(k, x, y); }, key, isA, j, mp, ax, ay, dist,
                        rs, sen_a);
         });
@@ -327,64 +331,265 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
             }
             d.b.pos[b] = p;
         }
+        la.contacts += count;
+        la.flags |= flags;
+    }
+    flush_accum(la, d.accum, d.block_contacts, 0, false);
+}
+
+constexpr uint32_t kColList = 16;    // corrections a collider keeps (the smallest keys)
+constexpr uint32_t kBodyMerge = 64;  // corrections a body merges in LDS
+
+// Contacts, first half of the two-kernel form: one thread per collider walks
+// its own stencil (physics.rs:248-300 for the pairs it is in), counts the
+// pairs it is A in, sends their events, records push-out rows, and keeps its
+// body's side of each correction -- the kColList smallest keys, in key
+// order -- in clist[i * nc + c], with the full count in ccnt[c].
+template <bool GROUPS, bool SENSORS>
+__global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d, SubstepArgs a) {
+    LocalAccum la;
+    la.init();
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) d.cfb_n[0] = 0u;  // k_contact_merge fills the fallback list after this kernel
+    if (c < d.nc) {
+        const int32_t b = d.c.parent[c];
+        const float2 xa = d.c.abs[c];
+        uint32_t count = 0, flags = 0;
+        ContactList<kColList, uint64_t> L;
+        L.init();
+        if (b >= 0 && finitef(xa.x) && finitef(xa.y)) {
+            const GridParams g = *d.grid;
+            const float D = read_dmax(d.accum);
+            const float ra = d.c.radius[c];
+            const float mown = d.b.mass[b];
+            uint2 ga = make_uint2(0, 0);
+            if (GROUPS) ga = d.c.groups[c];
+            const bool sen_a = SENSORS ? (d.c.cflags[c] & 1u) : false;
+            const float Q = ra + a.rmax + D;
+            auto enumerate = [&](auto&& emit) {
+                scan_stencil(g, d.cell_start, d.nc, d.crec, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, Q,
+                             [&](uint32_t t, const float4 (&rb)[4], uint32_t nv) {
+                    bool hit[4];
+                    float dx[4], dy[4], dist[4], rs[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const int32_t pj = __float_as_int(rb[u].w);
+                        dx[u] = xa.x - rb[u].x;
+                        dy[u] = xa.y - rb[u].y;
+                        rs[u] = ra + rb[u].z;
+                        const float dot = dx[u] * dx[u] + dy[u] * dy[u];
+                        dist[u] = sqrtf(dot);
+                        hit[u] = u < nv && pj >= 0 && pj != b &&
+                                 !(dot > (rs[u] * rs[u]) * 1.00000095367431640625f) && dist[u] < rs[u];
+                    }
+                    uint32_t jj[4];
+                    float mp[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        jj[u] = 0u;
+                        mp[u] = 0.0f;
+                        if (hit[u]) {
+                            jj[u] = d.vals[t + u];
+                            mp[u] = d.b.mass[__float_as_int(rb[u].w)];
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        if (!hit[u]) continue;
+                        const uint32_t j = jj[u];
+                        if (GROUPS) {
+                            uint2 gb = d.c.groups[j];
+                            if (!((ga.x & gb.y) != 0u && (gb.x & ga.y) != 0u)) continue;
+                        }
+                        const bool isA = c > j;
+                        const float ax = isA ? dx[u] : rb[u].x - xa.x, ay = isA ? dy[u] : rb[u].y - xa.y;
+                        const uint64_t key = isA ? (((uint64_t)c << 32) | j) : (((uint64_t)j << 32) | c);
+                        emit(key, isA, j, __float_as_int(rb[u].w), ax, ay, dist[u], rs[u], mp[u]);
+                    }
+                });
+            };
+            enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float ax, float ay, float dist, float rs,
+                          float mpart) {
+                if (isA) ++count;
+                // correction of this body's side (as k_contacts; a push-out only records its row)
+                float px = 0.0f, py = 0.0f;
+                const bool pushed = dist < 1e-6f;
+                if (pushed) {
+                    ax = ax + 0.02f;
+                    dist = vlen(ax, ay);
+                    px = isA ? 0.01f : -0.01f;
+                    py = isA ? 0.0f : -0.0f;
+                    flags |= INFO_PUSH_OUT;
+                    if (isA) {
+                        const uint32_t bit = 1u << (c & 31u);
+                        if (!(atomicOr(&d.ord_mark[c >> 5], bit) & bit)) d.ord_rows[atomicAdd(d.ord_n, 1u)] = c;
+                    }
+                }
+                const bool sen_b = SENSORS ? (d.c.cflags[j] & 1u) : false;
+                if (sen_a || sen_b) {
+                    if (pushed) L.add(key, px, py);
+                    return;
+                }
+                const float mA = isA ? mown : mpart, mB = isA ? mpart : mown;
+                const float ratio = 1.0f - mA / (mA + mB);
+                const float nx = ax / dist, ny = ay / dist;
+                if (nx != nx || ny != ny) flags |= ERR_NAN_NORMAL;
+                const float delta = rs - dist;
+                float cx, cy;
+                if (isA) {
+                    const float k = ratio * delta;
+                    cx = k * nx;
+                    cy = k * ny;
+                } else {
+                    const float k = (1.0f - ratio) * delta;
+                    cx = -(k * nx);
+                    cy = -(k * ny);
+                }
+                if (pushed) {
+                    cx = px + cx;
+                    cy = py + cy;
+                }
+                L.add(key, cx, cy);
+            });
+            if (d.events && count > 0) {
+                // collision events of the pairs whose A collider is this one (physics.rs:288-311)
+                unsigned long long e = atomicAdd(&d.accum->event_count, (unsigned long long)count);
+                const float2 va = d.b.vel[b];
+                enumerate([&](uint64_t key, bool isA, uint32_t j, int32_t pj, float, float, float, float, float) {
+                    if (!isA) return;
+                    if (e < d.event_cap) {
+                        DevEvent ev;
+                        ev.sub = a.sub_global;
+                        ev.a = (uint32_t)(key >> 32);
+                        ev.b = (uint32_t)key;
+                        ev.pad = 0;
+                        ev.va = va;
+                        ev.vb = d.b.vel[pj];
+                        d.events[e] = ev;
+                    }
+                    ++e;
+                });
+            }
+        }
+        const uint32_t nk = L.n < kColList ? L.n : kColList;
+#pragma unroll
+        for (uint32_t i = 0; i < kColList; ++i)
+            if (i < nk)
+                d.clist[(size_t)i * d.nc + c] =
+                    make_float4(__uint_as_float((uint32_t)(L.key[i] >> 32)), __uint_as_float((uint32_t)L.key[i]),
+                                L.cx[i], L.cy[i]);
+        d.ccnt[c] = L.n;
         la.contacts = count;
         la.flags = flags;
     }
     flush_accum(la, d.accum, d.block_contacts, 0, false);
 }
 
+// Second half: one thread per body gathers its colliders' corrections, sorts
+// them by key -- the reference's (i, j<i) order on this body -- and adds them
+// to the position.  A body with a collider past kColList corrections, or
+// more than kBodyMerge in all, goes to the fallback list (k_contacts in list
+// mode re-walks it).
+__global__ __launch_bounds__(kContactThreads) void k_contact_merge(DeviceWorld d) {
+    __shared__ float4 s_e[kBodyMerge][kContactThreads];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.nb) return;
+    float2 p = d.b.pos[b];
+    d.pos_pre[b] = p;  // the ordered pass re-sums from here if a push-out happens
+    const uint32_t k0 = d.bc_start[b], k1 = d.bc_start[b + 1];
+    uint32_t total = 0;
+    bool fallback = false;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t n = d.ccnt[d.bc_list[k]];
+        fallback |= n > kColList;
+        total += n;
+    }
+    if (fallback || total > kBodyMerge) {
+        d.cfb_list[atomicAdd(d.cfb_n, 1u)] = b;
+        return;
+    }
+    if (total == 0) return;
+    uint32_t m = 0;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t c = d.bc_list[k];
+        const uint32_t n = d.ccnt[c];
+        for (uint32_t i = 0; i < n; ++i) s_e[m++][lane] = d.clist[(size_t)i * d.nc + c];
+    }
+    // insertion sort by key (runs of one collider are already sorted)
+    for (uint32_t i = 1; i < m; ++i) {
+        const float4 v = s_e[i][lane];
+        const uint64_t kv = ((uint64_t)__float_as_uint(v.x) << 32) | __float_as_uint(v.y);
+        uint32_t j = i;
+        while (j > 0) {
+            const float4 u = s_e[j - 1][lane];
+            const uint64_t ku = ((uint64_t)__float_as_uint(u.x) << 32) | __float_as_uint(u.y);
+            if (ku <= kv) break;
+            s_e[j][lane] = u;
+            --j;
+        }
+        s_e[j][lane] = v;
+    }
+    for (uint32_t i = 0; i < m; ++i) {
+        const float4 v = s_e[i][lane];
+        p.x = p.x + v.z;
+        p.y = p.y + v.w;
+    }
+    d.b.pos[b] = p;
+}
+
 // One fixed joint (physics.rs:424-477) on body positions pa, pb and rotations
 // ra, rb (read before any write of this joint).  Returns false for coincident
 // anchors, which the reference skips; the caller then writes nothing.
-__device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, float& rb, float2 an_a, float2 an_b,
-                                            float jdist, float jtarget, float ma, float mb, bool sa, bool sb, float h,
-                                            uint32_t& flags) {
-    float wax = pa.x + an_a.x, way = pa.y + an_a.y;
-    float wbx = pb.x + an_b.x, wby = pb.y + an_b.y;
-    float dx = wbx - wax, dy = wby - way;
-    float dist = vlen(dx, dy);
-    if (dist < 1e-6f) return false;
-    float off_by = dist - jdist;
-    float cx = (off_by * dx) / dist, cy = (off_by * dy) / dist;
-    if (!(ma > 0.0f) || !(mb > 0.0f)) flags |= ERR_JOINT_MASS;
-    float inv_a = 1.0f / ma, inv_b = 1.0f / mb;
-    float ims = inv_a + inv_b;
-    if (sa) {
-        pb.x = pb.x - ims * cx;
-        pb.y = pb.y - ims * cy;
-    } else if (sb) {
-        pa.x = pa.x + ims * cx;
-        pa.y = pa.y + ims * cy;
-    } else {
-        float ratio = inv_a / ims;
-        float rbm = 1.0f - ratio;
-        pa.x = pa.x + ratio * cx;
-        pa.y = pa.y + ratio * cy;
-        pb.x = pb.x - rbm * cx;
-        pb.y = pb.y - rbm * cy;
-    }
+// The joint's mass terms (physics.rs:447-460) depend on the two bodies'
+// calculated_mass only, which no stage of a step changes: they come in
+// precomputed (DeviceWorld::j_mass, same f32 operations) and stay off the chain.
+// Rotation correction of one joint from its axis (physics.rs:462-469): the
+// reference adds corr * dt to rotation a and subtracts it from rotation b.
+// Positions never read rotations inside the joint stage (anchors are not
+// rotated, Q7), so these can be evaluated after the position sweep.
+__device__ __forceinline__ float joint_rotation_corr(float dx, float dy, float jtarget) {
     float angle_a = atan2_f32(dy, dx);
     float angle_b = -atan2_f32(dy, -dx);
     float diff = (angle_b - angle_a) - jtarget;
-    float corr = diff * 0.5f;
+    return diff * 0.5f;
+}
+
+__device__ __forceinline__ void joint_rotate(float& ra, float& rb, float corr, float h, uint32_t& flags) {
     ra = ra + corr * h;
     rb = rb - corr * h;
     if (!finitef(ra) || !finitef(rb)) flags |= ERR_JOINT_ROT;
+}
+
+// One fixed joint, positions then rotations.
+__device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, float& rb, float2 an_a, float2 an_b,
+                                            float jdist, float jtarget, float4 jm, bool sa, bool sb, float h,
+                                            uint32_t& flags) {
+    float dx, dy;
+    if (!joint_positions(pa, pb, an_a, an_b, jdist, jm, sa, sb, dx, dy, flags)) return false;
+    joint_rotate(ra, rb, joint_rotation_corr(dx, dy, jtarget), h, flags);
     return true;
 }
 
-// Fixed joints (physics.rs:424-477): Gauss-Seidel in arena order inside each
-// component of the joint graph; components touch disjoint bodies, so running
-// them concurrently reproduces the sequential sweep.  One thread per
-// component.  A component of at most kJointBodies bodies and kJointJoints
-// joints iterates on copies of its bodies and joints in LDS (the sweep is a
-// chain of dependent reads and writes, so each step costs an LDS round trip
-// instead of a cache round trip) and writes its bodies back once; a larger
-// one iterates in global memory.  Writes keep the reference's order (a's
-// position, b's, a's rotation, b's), which decides a joint of a body to itself.
-__global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uint32_t iterations) {
+// Fixed joints (physics.rs:424-477), positions: Gauss-Seidel in arena order
+// inside each connected component of the joint graph (components touch
+// disjoint bodies, so they run in parallel, one lane each).  A component of
+// at most kJointBodies bodies and kJointJoints joints iterates on copies of
+// its bodies and joints in LDS (the sweep is a chain of dependent reads and
+// writes, so each step costs an LDS round trip instead of a cache round trip)
+// and writes its bodies back once; a larger one iterates in global memory.
+// Writes keep the reference's order (a's position, b's, a's rotation, b's),
+// which decides a joint of a body to itself.  With `defer`,
+// every solve's axis goes to jrec[iterations * jc_start[c] + it * nq + q] --
+// record (dx, dy, target angle, 1), or 0 when the anchors coincide and the
+// reference skips the joint -- and the rotations are applied later:
+// k_joint_corr evaluates the atan2 pairs of all records at once and
+// k_integrate adds each body's corrections in sequence order.  Without it
+// (joint_iterations changed since the upload) the rotations are applied here.
+__global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uint32_t iterations, bool defer) {
     __shared__ float2 s_pos[kJointBodies][64];
-    __shared__ float s_rot[kJointBodies][64], s_mass[kJointBodies][64];
+    __shared__ float s_rot[kJointBodies][64];
+    __shared__ float4 s_jm[kJointJoints][64];      // mass terms of the joint
     __shared__ int32_t s_gid[kJointBodies][64];    // body index
     __shared__ float4 s_anchor[kJointJoints][64];  // anchor a, anchor b
     __shared__ float2 s_len[kJointJoints][64];     // distance, target angle
@@ -395,6 +600,8 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
     if (comp < d.n_jcomp) {
         const uint32_t k0 = d.jc_start[comp], k1 = d.jc_start[comp + 1];
         const uint32_t b0 = d.jb_start[comp], nbl = d.jb_start[comp + 1] - b0;
+        const uint32_t nq = k1 - k0;
+        float4* rec = defer ? d.jrec + (size_t)iterations * k0 : nullptr;
         if (nbl > 0) {
             // copies in, eight entries per round so that their loads overlap (an index past
             // the end repeats the last entry, rewriting its slot with the same values)
@@ -408,12 +615,10 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
                     const uint32_t l = min(l0 + u, nbl - 1);
                     s_gid[l][lane] = g[u];
                     s_pos[l][lane] = d.b.pos[g[u]];
-                    s_rot[l][lane] = d.b.rot[g[u]];
-                    s_mass[l][lane] = d.b.mass[g[u]];
+                    if (!defer) s_rot[l][lane] = d.b.rot[g[u]];
                     if (d.b.bflags[g[u]] & BF_STATIC) stat |= 1u << l;
                 }
             }
-            const uint32_t nq = k1 - k0;
             for (uint32_t q0 = 0; q0 < nq; q0 += 8) {
                 uint32_t j[8];
 #pragma unroll
@@ -424,6 +629,7 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
                     const float2 an_a = d.j_anchor_a[j[u]], an_b = d.j_anchor_b[j[u]];
                     s_anchor[q][lane] = make_float4(an_a.x, an_a.y, an_b.x, an_b.y);
                     s_len[q][lane] = make_float2(d.j_dist[j[u]], d.j_target[j[u]]);
+                    s_jm[q][lane] = d.j_mass[j[u]];
                     s_loc[q][lane] = d.j_local[k0 + q];
                 }
             }
@@ -434,21 +640,26 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
                     const float4 an = s_anchor[q][lane];
                     const float2 len = s_len[q][lane];
                     float2 pa = s_pos[la][lane], pb = s_pos[lb][lane];
-                    float ra = s_rot[la][lane], rb = s_rot[lb][lane];
-                    if (!joint_solve(pa, pb, ra, rb, make_float2(an.x, an.y), make_float2(an.z, an.w), len.x, len.y,
-                                     s_mass[la][lane], s_mass[lb][lane], (stat >> la) & 1u, (stat >> lb) & 1u, a.h,
-                                     flags))
-                        continue;
+                    float dx, dy;
+                    const bool moved = joint_positions(pa, pb, make_float2(an.x, an.y), make_float2(an.z, an.w),
+                                                       len.x, s_jm[q][lane], (stat >> la) & 1u, (stat >> lb) & 1u,
+                                                       dx, dy, flags);
+                    if (defer) rec[it * nq + q] = moved ? make_float4(dx, dy, len.y, 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (!moved) continue;
                     s_pos[la][lane] = pa;
                     s_pos[lb][lane] = pb;
-                    s_rot[la][lane] = ra;
-                    s_rot[lb][lane] = rb;
+                    if (!defer) {
+                        float ra = s_rot[la][lane], rb = s_rot[lb][lane];
+                        joint_rotate(ra, rb, joint_rotation_corr(dx, dy, len.y), a.h, flags);
+                        s_rot[la][lane] = ra;
+                        s_rot[lb][lane] = rb;
+                    }
                 }
             }
             for (uint32_t l = 0; l < nbl; ++l) {
                 const int32_t g = s_gid[l][lane];
                 d.b.pos[g] = s_pos[l][lane];
-                d.b.rot[g] = s_rot[l][lane];
+                if (!defer) d.b.rot[g] = s_rot[l][lane];
             }
         } else {
             for (uint32_t it = 0; it < iterations; ++it) {
@@ -456,20 +667,37 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
                     const uint32_t j = d.jc_list[kk];
                     const int32_t ia = d.j_a[j], ib = d.j_b[j];
                     float2 pa = d.b.pos[ia], pb = d.b.pos[ib];
-                    float ra = d.b.rot[ia], rb = d.b.rot[ib];
-                    if (!joint_solve(pa, pb, ra, rb, d.j_anchor_a[j], d.j_anchor_b[j], d.j_dist[j], d.j_target[j],
-                                     d.b.mass[ia], d.b.mass[ib], d.b.bflags[ia] & BF_STATIC,
-                                     d.b.bflags[ib] & BF_STATIC, a.h, flags))
-                        continue;
+                    float dx, dy;
+                    const bool moved =
+                        joint_positions(pa, pb, d.j_anchor_a[j], d.j_anchor_b[j], d.j_dist[j], d.j_mass[j],
+                                        d.b.bflags[ia] & BF_STATIC, d.b.bflags[ib] & BF_STATIC, dx, dy, flags);
+                    if (defer)
+                        rec[it * nq + (kk - k0)] =
+                            moved ? make_float4(dx, dy, d.j_target[j], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (!moved) continue;
                     d.b.pos[ia] = pa;
                     d.b.pos[ib] = pb;
-                    d.b.rot[ia] = ra;
-                    d.b.rot[ib] = rb;
+                    if (!defer) {
+                        float ra = d.b.rot[ia], rb = d.b.rot[ib];
+                        joint_rotate(ra, rb, joint_rotation_corr(dx, dy, d.j_target[j]), a.h, flags);
+                        d.b.rot[ia] = ra;
+                        d.b.rot[ib] = rb;
+                    }
                 }
             }
         }
     }
     if (flags) atomicOr(&d.accum->error_flags, flags);
+}
+
+// Rotation correction of every joint solve of the substep (physics.rs:462-469),
+// all at once: (corr, 1) for a solve that moved its bodies, (0, 0) for a skipped one.
+__global__ __launch_bounds__(kThreads) void k_joint_corr(const float4* __restrict__ rec, uint32_t n,
+                                                         float2* __restrict__ corr) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float4 x = rec[r];
+    corr[r] = x.w != 0.0f ? make_float2(joint_rotation_corr(x.x, x.y, x.z), 1.0f) : make_float2(0.0f, 0.0f);
 }
 
 // Verlet + collider transforms + constraints (physics.rs:323-395).
@@ -483,9 +711,30 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
         s.p = d.b.pos[b];
         s.rot = d.b.rot[b];
         s.omega = d.b.omega[b];
+        const float rot_in = s.rot;
+        if (a.joint_rot) {
+            // this body's joint rotation corrections (physics.rs:468-469), in the order the
+            // joint stage made them: iterations, then the component's joints in arena order
+            const int32_t c = d.bj_comp[b];
+            if (c >= 0) {
+                const uint32_t k0 = d.jc_start[c], nq = d.jc_start[c + 1] - k0;
+                const uint32_t e0 = d.bj_start[b], e1 = d.bj_start[b + 1];
+                const float2* corr = d.jcorr + (size_t)a.joint_iters * k0;
+                for (uint32_t it = 0; it < a.joint_iters; ++it)
+                    for (uint32_t e = e0; e < e1; ++e) {
+                        const uint32_t ent = d.bj_list[e];
+                        const float2 cr = corr[it * nq + ((ent >> 1) - k0)];
+                        if (cr.y == 0.0f) continue;
+                        const float dlt = cr.x * a.h;
+                        s.rot = (ent & 1u) ? s.rot - dlt : s.rot + dlt;
+                        if (!finitef(s.rot)) la.flags |= ERR_JOINT_ROT;
+                    }
+            }
+        }
         if (fl & BF_STATIC) {  // :327-332
             d.b.pos_old[b] = s.p;
             d.b.vel[b] = make_float2(0.0f, 0.0f);
+            if (__float_as_uint(s.rot) != __float_as_uint(rot_in)) d.b.rot[b] = s.rot;
         } else {
             float2 acc;
             if (a.has_springs) {
@@ -506,11 +755,11 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
                 ratio = a.h / d.accum->old_dt;
                 d.accum->old_dt = a.h;
             }
-            const float rot0 = s.rot, om0 = s.omega;
+            const float om0 = s.omega;
             verlet_dynamic(s, acc, toi, vr, vreq, a.h, ratio);
             d.b.pos_old[b] = s.po;
             d.b.vel[b] = s.vel;
-            if (__float_as_uint(s.rot) != __float_as_uint(rot0)) d.b.rot[b] = s.rot;
+            if (__float_as_uint(s.rot) != __float_as_uint(rot_in)) d.b.rot[b] = s.rot;
             if (__float_as_uint(s.omega) != __float_as_uint(om0)) d.b.omega[b] = s.omega;
             if (vr) d.b.bflags[b] = fl & ~BF_HAS_VREQ;
             if (a.fresh) d.b.torque[b] = 0.0f;
@@ -572,25 +821,48 @@ hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t
     const uint32_t blocks = div_up(d.nb, kThreads);
     if (a.has_springs) k_forces<<<blocks, kThreads, 0, d.stream>>>(d, a);
     if (a.collisions && d.nc > 0) {
-        // one-wave blocks: a body's work varies with its collider count, so small blocks
-        // keep the slow waves from holding whole workgroups
+        // One-wave blocks: a body's work varies with its collider count, so small blocks
+        // keep the slow waves from holding whole workgroups.  With the collider lists
+        // allocated: one thread per collider, a per-body merge, and the full per-body
+        // walk only for the bodies the merge could not take.
         const uint32_t cb = div_up(d.nb, kContactThreads);
+#define BG_CONTACTS(G, S)                                                                                     \
+    do {                                                                                                      \
+        if (d.clist) {                                                                                        \
+            k_contact_lists<G, S><<<div_up(d.nc, kContactThreads), kContactThreads, 0, d.stream>>>(d, a);     \
+            k_contact_merge<<<cb, kContactThreads, 0, d.stream>>>(d);                                         \
+            k_contacts<G, S><<<std::min(cb, 256u), kContactThreads, 0, d.stream>>>(d, a, d.cfb_list);         \
+        } else {                                                                                              \
+            k_contacts<G, S><<<cb, kContactThreads, 0, d.stream>>>(d, a, nullptr);                            \
+        }                                                                                                     \
+    } while (0)
         if (groups && sensors)
-            k_contacts<true, true><<<cb, kContactThreads, 0, d.stream>>>(d, a);
+            BG_CONTACTS(true, true);
         else if (groups)
-            k_contacts<true, false><<<cb, kContactThreads, 0, d.stream>>>(d, a);
+            BG_CONTACTS(true, false);
         else if (sensors)
-            k_contacts<false, true><<<cb, kContactThreads, 0, d.stream>>>(d, a);
+            BG_CONTACTS(false, true);
         else
-            k_contacts<false, false><<<cb, kContactThreads, 0, d.stream>>>(d, a);
+            BG_CONTACTS(false, false);
+#undef BG_CONTACTS
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         e = launch_contacts_ordered(d, a, groups, sensors);
         if (e != hipSuccess) return e;
     }
-    if (d.n_jcomp > 0 && joint_iterations > 0)
-        k_joints<<<div_up(d.n_jcomp, 64), 64, 0, d.stream>>>(d, a, joint_iterations);
-    k_integrate<<<blocks, kThreads, 0, d.stream>>>(d, a);
+    SubstepArgs ai = a;
+    ai.joint_rot = 0u;
+    if (d.n_jcomp > 0 && joint_iterations > 0) {
+        const bool defer = d.jrec && joint_iterations == d.j_iters;
+        k_joints<<<div_up(d.n_jcomp, 64), 64, 0, d.stream>>>(d, a, joint_iterations, defer);
+        if (defer) {
+            const uint32_t nrec = joint_iterations * d.nj;
+            k_joint_corr<<<div_up(nrec, kThreads), kThreads, 0, d.stream>>>(d.jrec, nrec, d.jcorr);
+            ai.joint_rot = 1u;
+            ai.joint_iters = joint_iterations;
+        }
+    }
+    k_integrate<<<blocks, kThreads, 0, d.stream>>>(d, ai);
     return hipGetLastError();
 }
 

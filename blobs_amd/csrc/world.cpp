@@ -700,13 +700,14 @@ void World::free_arrays() {
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
+    dfree(d.j_mass), dfree(d.jrec), dfree(d.jcorr), dfree(d.bj_start), dfree(d.bj_list), dfree(d.bj_comp);
     dfree(d.jc_start), dfree(d.jc_list), dfree(d.jb_start), dfree(d.jb_list), dfree(d.j_local), dfree(d.con_pos),
         dfree(d.con_r);
     dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.crec);
     dfree(d.grid), dfree(d.accum), dfree(d.block_contacts), dfree(d.events), dfree(d.vel_next);
     dfree(d.ord_rows), dfree(d.ord_n), dfree(d.ord_mark), dfree(d.pos_pre), dfree(d.mv_key), dfree(d.mv_abs);
     dfree(d.mv_list), dfree(d.ord_heap), dfree(d.ord_log), dfree(d.aff_list), dfree(d.aff_mark), dfree(d.ord_cand);
-    dfree(d.ord_terms);
+    dfree(d.ord_terms), dfree(d.clist), dfree(d.ccnt), dfree(d.cfb_list), dfree(d.cfb_n);
     sort_temp_free(d.sort);
     hipStream_t s = d.stream;
     int dev = d.device;
@@ -922,6 +923,15 @@ int World::upload() {
         j_dist.push_back(j.distance);
         j_target.push_back(j.target_angle);
     }
+    // mass terms of each joint (physics.rs:447-460), from the masses of this upload
+    std::vector<float4> j_mass(nj);
+    for (uint32_t k = 0; k < nj; ++k) {
+        const float ma = mass[j_a[k]], mb = mass[j_b[k]];
+        const float inv_a = 1.0f / ma, inv_b = 1.0f / mb;
+        const float ims = inv_a + inv_b;
+        const float ratio = inv_a / ims;
+        j_mass[k] = make_float4(ims, ratio, 1.0f - ratio, (ma > 0.0f && mb > 0.0f) ? 1.0f : 0.0f);
+    }
     std::vector<uint32_t> jc_start, jc_list, jb_start, j_local;
     std::vector<int32_t> jb_list;
     uint32_t n_jcomp = 0;
@@ -978,6 +988,25 @@ int World::upload() {
             }
             jb_start[c + 1] = (uint32_t)jb_list.size();
         }
+    }
+    // per body: its joints in arena order and its component (deferred joint rotations)
+    std::vector<uint32_t> bj_start(nb + 1, 0), bj_list;
+    std::vector<int32_t> bj_comp(nb, -1);
+    if (n_jcomp > 0) {
+        for (uint32_t k = 0; k < nj; ++k) {
+            bj_start[j_a[k] + 1]++;
+            bj_start[j_b[k] + 1]++;
+        }
+        for (uint32_t i = 0; i < nb; ++i) bj_start[i + 1] += bj_start[i];
+        bj_list.resize(bj_start[nb]);
+        std::vector<uint32_t> fillb(bj_start.begin(), bj_start.end() - 1);
+        for (uint32_t c = 0; c < n_jcomp; ++c)
+            for (uint32_t kk = jc_start[c]; kk < jc_start[c + 1]; ++kk) {
+                const uint32_t j = jc_list[kk];
+                bj_list[fillb[j_a[j]]++] = kk << 1;
+                bj_list[fillb[j_b[j]]++] = (kk << 1) | 1u;
+                bj_comp[j_a[j]] = bj_comp[j_b[j]] = (int32_t)c;
+            }
     }
 
     // constraints
@@ -1210,6 +1239,15 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.j_anchor_b, nj));
     HIP_TRY(dmalloc(&d.j_dist, nj));
     HIP_TRY(dmalloc(&d.j_target, nj));
+    HIP_TRY(dmalloc(&d.j_mass, nj));
+    if (n_jcomp > 0 && params_.joint_iterations > 0) {
+        d.j_iters = params_.joint_iterations;
+        HIP_TRY(dmalloc(&d.jrec, (size_t)d.j_iters * nj));
+        HIP_TRY(dmalloc(&d.jcorr, (size_t)d.j_iters * nj));
+        HIP_TRY(dmalloc(&d.bj_start, nb + 1));
+        HIP_TRY(dmalloc(&d.bj_list, bj_list.size()));
+        HIP_TRY(dmalloc(&d.bj_comp, nb));
+    }
     HIP_TRY(dmalloc(&d.jc_start, n_jcomp + 1));
     HIP_TRY(dmalloc(&d.jc_list, nj));
     HIP_TRY(dmalloc(&d.jb_start, n_jcomp + 1));
@@ -1246,6 +1284,12 @@ int World::upload() {
         HIP_TRY(hipMemsetAsync(d.ord_mark, 0, sizeof(uint32_t) * ((ncw + 31) / 32), s));
         HIP_TRY(hipMemsetAsync(d.aff_mark, 0, sizeof(uint32_t) * ((nbw + 31) / 32), s));
         HIP_TRY(hipMemsetAsync(d.mv_key, 0xFF, sizeof(unsigned long long) * ncw, s));
+        // two-kernel contacts
+        HIP_TRY(dmalloc(&d.clist, (size_t)16 * ncw));
+        HIP_TRY(dmalloc(&d.ccnt, ncw));
+        HIP_TRY(dmalloc(&d.cfb_list, nbw));
+        HIP_TRY(dmalloc(&d.cfb_n, 1));
+        HIP_TRY(hipMemsetAsync(d.cfb_n, 0, sizeof(uint32_t), s));
     }
     HIP_TRY(dmalloc(&d.grid, 1));
     HIP_TRY(dmalloc(&d.accum, 1));
@@ -1255,7 +1299,7 @@ int World::upload() {
         if (sphere) HIP_TRY(dmalloc(&d.vel_next, nb));
     }
     // one contact slot per block of any per-body launch (the smallest blocks have 64 threads)
-    d.n_block_slots = (std::max<uint32_t>(nb, 1) + 63) / 64;
+    d.n_block_slots = (std::max<uint32_t>(std::max(nb, nc), 1) + 63) / 64;  // per-body and per-collider launches
     HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
     HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
     HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, cell_rows + 1)));
@@ -1277,6 +1321,12 @@ int World::upload() {
     HIP_TRY(h2d(d.j_anchor_b, j_an_b, s));
     HIP_TRY(h2d(d.j_dist, j_dist, s));
     HIP_TRY(h2d(d.j_target, j_target, s));
+    HIP_TRY(h2d(d.j_mass, j_mass, s));
+    if (d.jrec) {
+        HIP_TRY(h2d(d.bj_start, bj_start, s));
+        HIP_TRY(h2d(d.bj_list, bj_list, s));
+        HIP_TRY(h2d(d.bj_comp, bj_comp, s));
+    }
     HIP_TRY(h2d(d.jc_start, jc_start, s));
     HIP_TRY(h2d(d.jc_list, jc_list, s));
     HIP_TRY(h2d(d.jb_start, jb_start, s));
@@ -1425,7 +1475,7 @@ int World::rebuild() {
 
 int World::run_substep(float h, bool fresh, bool last) {
     DeviceWorld& d = d_;
-    SubstepArgs a;
+    SubstepArgs a{};
     a.h = h;
     a.ratio_rest = h / h;
     a.gx = params_.gravity[0];
