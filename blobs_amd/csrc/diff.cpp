@@ -43,7 +43,6 @@ hipError_t up(T* dst, const std::vector<T>& v, hipStream_t s) {
 
 int World::export_diff(DiffHostWorld& out) {
     if (sync_host()) return fail(BG_ERR_HIP, "download failed");
-    if (joints_.len() != 0) return fail(BG_ERR_UNSUPPORTED, "differentiable rollouts take worlds without joints");
     if (!constraints_.empty())
         return fail(BG_ERR_UNSUPPORTED, "differentiable rollouts take worlds without circle constraints");
     if (params_.collisions_enabled && params_.use_spatial_hash)
@@ -110,6 +109,22 @@ int World::export_diff(DiffHostWorld& out) {
         out.k.push_back(sp.stiffness);
         out.c.push_back(sp.damping);
     }
+    for (uint32_t s = 0; s < joints_.capacity_slots(); ++s) {  // arena order (physics.rs:426)
+        if (!joints_.occupied(s)) continue;
+        const bg_fixed_joint& j = joints_.at_slot(s);
+        const HostBody* a = bodies_.get(j.rigid_body_a);
+        const HostBody* b = bodies_.get(j.rigid_body_b);
+        if (!a || !b || j.rigid_body_a == j.rigid_body_b)
+            return fail(BG_ERR_PANIC, "joint bodies must exist and differ (physics.rs:427-432 unwrap)");
+        if (!(a->f.calculated_mass > 0.0f) || !(b->f.calculated_mass > 0.0f))
+            return fail(BG_ERR_PANIC, "joint bodies need a positive mass (physics.rs:447-448 assert)");
+        out.ja.push_back(dense_of_slot[Arena<HostBody>::slot_of(j.rigid_body_a)]);
+        out.jb.push_back(dense_of_slot[Arena<HostBody>::slot_of(j.rigid_body_b)]);
+        out.jan_a.push_back(make_float2(j.anchor_a[0], j.anchor_a[1]));
+        out.jan_b.push_back(make_float2(j.anchor_b[0], j.anchor_b[1]));
+        out.jdist.push_back(j.distance);
+    }
+    out.joint_iterations = params_.joint_iterations;
     out.old_dt = params_.old_dt;
     out.gx = params_.gravity[0];
     out.gy = params_.gravity[1];
@@ -128,6 +143,8 @@ void Rollout::release() {
     dfree(d_ck_p_), dfree(d_ck_q_), dfree(d_ck_v_), dfree(d_final_), dfree(d_loss_);
     dfree(d_tape_s_), dfree(d_tape_nbr_), dfree(d_lam_p_), dfree(d_lam_q_), dfree(d_dl_), dfree(d_gv_),
         dfree(d_gk_);
+    dfree(d_tape_v_), dfree(d_tape_pj_), dfree(d_j_axis_), dfree(d_lam_v_), dfree(d_jc_start_), dfree(d_j_ab_),
+        dfree(d_j_anchor_), dfree(d_j_mass_), dfree(d_j_dist_);
     if (stream_) (void)hipStreamDestroy(stream_);
     stream_ = nullptr;
     n_ck_alloc_ = tape_sub_alloc_ = 0;
@@ -143,6 +160,9 @@ int Rollout::create(World* const* worlds, uint32_t n, const float* targets, uint
     std::vector<DiffWorld> ws(n);
     std::vector<float2> p0, q0, v0;
     std::vector<uint32_t> inc_count;
+    std::vector<uint32_t> jc_start(1, 0), j_ab;
+    std::vector<float4> j_anchor, j_mass;
+    std::vector<float> j_dist;
     damped_ = false;
     for (uint32_t w = 0; w < n; ++w) {
         if (!worlds[w]) return fail(BG_ERR_INVALID_ARGUMENT, "null world");
@@ -154,8 +174,11 @@ int Rollout::create(World* const* worlds, uint32_t n, const float* targets, uint
             gy_ = h.gy;
             substeps_ = h.substeps;
             collisions_ = h.collisions;
-        } else if (h.gx != gx_ || h.gy != gy_ || h.substeps != substeps_ || h.collisions != collisions_) {
-            return fail(BG_ERR_INVALID_ARGUMENT, "worlds of a rollout share gravity, substeps and collisions_enabled");
+            j_iters_ = h.joint_iterations;
+        } else if (h.gx != gx_ || h.gy != gy_ || h.substeps != substeps_ || h.collisions != collisions_ ||
+                   h.joint_iterations != j_iters_) {
+            return fail(BG_ERR_INVALID_ARGUMENT,
+                        "worlds of a rollout share gravity, substeps, joint_iterations and collisions_enabled");
         }
         DiffWorld& d = ws[w];
         d.b0 = (uint32_t)mass_.size();
@@ -180,6 +203,47 @@ int Rollout::create(World* const* worlds, uint32_t n, const float* targets, uint
         radius_.insert(radius_.end(), h.radius.begin(), h.radius.end());
         gmod_.insert(gmod_.end(), h.gmod.begin(), h.gmod.end());
         flags_.insert(flags_.end(), h.flags.begin(), h.flags.end());
+        // joints: connected components of the world's joint graph (union-find over bodies), each
+        // component's joints contiguous and in arena order; components touch disjoint bodies, so
+        // sweeping them concurrently is the reference's sequential sweep
+        d.jc0 = (uint32_t)jc_start.size() - 1;
+        d.njc = 0;
+        if (!h.ja.empty()) {
+            std::vector<int32_t> uf(d.nb);
+            for (uint32_t i = 0; i < d.nb; ++i) uf[i] = (int32_t)i;
+            auto find = [&](int32_t x) {
+                while (uf[x] != x) x = uf[x] = uf[uf[x]];
+                return x;
+            };
+            for (size_t q = 0; q < h.ja.size(); ++q) {
+                const int32_t ra = find(h.ja[q]), rb = find(h.jb[q]);
+                if (ra != rb) uf[ra] = rb;
+            }
+            std::vector<int32_t> comp_of_root(d.nb, -1);
+            std::vector<std::vector<uint32_t>> comps;
+            for (size_t q = 0; q < h.ja.size(); ++q) {
+                const int32_t r = find(h.ja[q]);
+                if (comp_of_root[r] < 0) {
+                    comp_of_root[r] = (int32_t)comps.size();
+                    comps.emplace_back();
+                }
+                comps[comp_of_root[r]].push_back((uint32_t)q);
+            }
+            for (const auto& cq : comps) {
+                for (uint32_t q : cq) {
+                    const float ma = h.mass[h.ja[q]], mb = h.mass[h.jb[q]];
+                    const float inv_a = 1.0f / ma, inv_b = 1.0f / mb;  // physics.rs:450-455
+                    const float ims = inv_a + inv_b;
+                    const float ratio = inv_a / ims;
+                    j_ab.push_back((uint32_t)h.ja[q] | ((uint32_t)h.jb[q] << 16));
+                    j_anchor.push_back(make_float4(h.jan_a[q].x, h.jan_a[q].y, h.jan_b[q].x, h.jan_b[q].y));
+                    j_dist.push_back(h.jdist[q]);
+                    j_mass.push_back(make_float4(ims, ratio, 1.0f - ratio, 1.0f));
+                }
+                jc_start.push_back((uint32_t)j_ab.size());
+            }
+            d.njc = (uint32_t)comps.size();
+        }
         for (size_t s = 0; s < h.k.size(); ++s) {
             sa_.push_back((int32_t)d.b0 + h.sa[s]);
             sb_.push_back((int32_t)d.b0 + h.sb[s]);
@@ -191,6 +255,7 @@ int Rollout::create(World* const* worlds, uint32_t n, const float* targets, uint
     }
     nb_ = (uint32_t)mass_.size();
     ns_ = (uint32_t)k_.size();
+    nj_ = (uint32_t)j_ab.size();
     n_worlds_ = n;
     body0_.resize(n + 1);
     for (uint32_t w = 0; w < n; ++w) body0_[w] = ws[w].b0;
@@ -234,6 +299,18 @@ int Rollout::create(World* const* worlds, uint32_t n, const float* targets, uint
     HIP_TRY(dalloc(&d_dl_, nb_));
     HIP_TRY(dalloc(&d_gv_, nb_));
     HIP_TRY(dalloc(&d_gk_, ns_));
+    HIP_TRY(dalloc(&d_lam_v_, nb_));
+    HIP_TRY(dalloc(&d_jc_start_, jc_start.size()));
+    HIP_TRY(dalloc(&d_j_ab_, nj_));
+    HIP_TRY(dalloc(&d_j_anchor_, nj_));
+    HIP_TRY(dalloc(&d_j_mass_, nj_));
+    HIP_TRY(dalloc(&d_j_dist_, nj_));
+    HIP_TRY(dalloc(&d_j_axis_, (size_t)nj_ * (j_iters_ ? j_iters_ : 1)));
+    HIP_TRY(up(d_jc_start_, jc_start, s));
+    HIP_TRY(up(d_j_ab_, j_ab, s));
+    HIP_TRY(up(d_j_anchor_, j_anchor, s));
+    HIP_TRY(up(d_j_mass_, j_mass, s));
+    HIP_TRY(up(d_j_dist_, j_dist, s));
     HIP_TRY(up(d_worlds_, ws, s));
     HIP_TRY(up(d_mass_, mass_, s));
     HIP_TRY(up(d_radius_, radius_, s));
@@ -281,9 +358,11 @@ int Rollout::ensure_buffers(uint32_t n_steps) {
     }
     const uint32_t seg_sub = (n_steps < ck_every_ ? n_steps : ck_every_) * substeps_;
     if (seg_sub > tape_sub_alloc_) {
-        dfree(d_tape_s_), dfree(d_tape_nbr_);
+        dfree(d_tape_s_), dfree(d_tape_nbr_), dfree(d_tape_v_), dfree(d_tape_pj_);
         HIP_TRY(dalloc(&d_tape_s_, (size_t)seg_sub * nb_));
         HIP_TRY(dalloc(&d_tape_nbr_, (size_t)seg_sub * nb_ * kDiffTapeNbr));
+        if (damped_) HIP_TRY(dalloc(&d_tape_v_, (size_t)seg_sub * nb_));
+        if (nj_ > 0 && j_iters_ > 0) HIP_TRY(dalloc(&d_tape_pj_, (size_t)seg_sub * nb_));
         tape_sub_alloc_ = seg_sub;
     }
     return BG_OK;
@@ -308,6 +387,13 @@ DiffArgs Rollout::args() const {
     a.sp_c = d_c_;
     a.inc_start = d_inc_start_;
     a.inc_list = d_inc_list_;
+    a.jc_start = d_jc_start_;
+    a.j_ab = d_j_ab_;
+    a.j_anchor = d_j_anchor_;
+    a.j_dist = d_j_dist_;
+    a.j_mass = d_j_mass_;
+    a.n_joints = nj_;
+    a.j_iters = nj_ > 0 ? j_iters_ : 0u;
     a.ck_p = d_ck_p_;
     a.ck_q = d_ck_q_;
     a.ck_v = d_ck_v_;
@@ -315,8 +401,12 @@ DiffArgs Rollout::args() const {
     a.loss = d_loss_;
     a.tape_s = d_tape_s_;
     a.tape_nbr = d_tape_nbr_;
+    a.tape_v = d_tape_v_;
+    a.tape_pj = d_tape_pj_;
+    a.j_axis = d_j_axis_;
     a.lam_p = d_lam_p_;
     a.lam_q = d_lam_q_;
+    a.lam_v = d_lam_v_;
     a.dl_dfinal = nullptr;
     a.grad_vreq = d_gv_;
     a.grad_k = d_gk_;
@@ -349,7 +439,6 @@ int Rollout::forward(double delta, uint32_t n_steps, float* loss_per_world) {
 
 int Rollout::backward(const float* dl_dfinal, float* grad_vreq, float* grad_k) {
     if (!have_forward_) return fail(BG_ERR_INVALID_ARGUMENT, "backward needs a forward with the current parameters");
-    if (damped_) return fail(BG_ERR_UNSUPPORTED, "the adjoint covers undamped springs only (damping 0)");
     DiffArgs a = args();
     if (dl_dfinal) {
         std::vector<float2> dl(nb_);

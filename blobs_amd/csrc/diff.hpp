@@ -1,13 +1,16 @@
 // diff.hpp -- differentiable rollouts of batches of single-sphere worlds with
-// springs: forward with checkpoints, adjoint by segment recomputation.
+// springs (damped or not) and fixed joints: forward with checkpoints, adjoint
+// by segment recomputation.
 //
 // Each world runs in one 1024-thread workgroup for the whole rollout (two
 // bodies per thread, all shared state in LDS), so a world holds at most
 // kDiffMaxBodies bodies.  The forward is the reference substep
-// (physics.rs:401-421: gravity, springs, contacts, Verlet, collider
-// transform) with the same f32 operation order as the general path; the
-// adjoint differentiates it with the contact pair set of every substep held
-// fixed (recomputed from the recorded snapshot, not re-decided).
+// (physics.rs:401-421: gravity, springs, contacts, joint positions, Verlet,
+// collider transform) with the same f32 operation order as the general
+// path; the adjoint differentiates it with the contact pair set of every
+// substep held fixed (recomputed from the recorded snapshot, not re-decided).
+// Joint rotations are not part of the rollout state: with centred colliders
+// and unrotated anchors no position reads a rotation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,6 +36,7 @@ struct DiffWorld {
     int32_t first_dyn;  // local index of the first non-static body, -1 if none
     float rmax;         // largest radius of the world
     float tx, ty;       // target point of the centroid loss
+    uint32_t jc0, njc;  // joint components [jc0, jc0 + njc) (their joints are contiguous)
 };
 
 struct DiffArgs {
@@ -53,6 +57,13 @@ struct DiffArgs {
     const float* sp_c;
     const uint32_t* inc_start;  // per body: its springs in ascending index
     const uint32_t* inc_list;
+    // fixed joints, grouped by connected component (arena order inside one)
+    const uint32_t* jc_start;  // per component: first joint, n_components + 1 entries
+    const uint32_t* j_ab;      // local body a | local body b << 16
+    const float4* j_anchor;    // anchor a (x, y), anchor b (z, w)
+    const float* j_dist;
+    const float4* j_mass;  // 1/m_a + 1/m_b, ratio, 1 - ratio, unused
+    uint32_t n_joints, j_iters;
     // checkpoints [n_ck][nb_total] of position, position_old, calculated_velocity
     float2* ck_p;
     float2* ck_q;
@@ -62,9 +73,13 @@ struct DiffArgs {
     // tape of one segment, [substep in segment][nb_total]
     float2* tape_s;
     uint16_t* tape_nbr;  // [..][nb_total][kDiffTapeNbr]
+    float2* tape_v;      // calculated_velocity at the substep start (damped springs), else null
+    float2* tape_pj;     // positions entering the joint stage (joints), else null
+    float2* j_axis;      // [j_iters][n_joints]: anchor axes of the reverse pass's joint sweep
     // adjoint
     float2* lam_p;  // d loss / d position at a segment boundary
     float2* lam_q;  // d loss / d position_old
+    float2* lam_v;  // d loss / d calculated_velocity
     const float2* dl_dfinal;  // d loss / d final position, or null for the centroid loss
     float2* grad_vreq;
     float* grad_k;
@@ -107,7 +122,7 @@ private:
 
     hipStream_t stream_ = nullptr;
     uint32_t n_worlds_ = 0, nb_ = 0, ns_ = 0, substeps_ = 8, ck_every_ = 20, n_steps_ = 0;
-    uint32_t n_ck_alloc_ = 0, tape_sub_alloc_ = 0;
+    uint32_t n_ck_alloc_ = 0, tape_sub_alloc_ = 0, nj_ = 0, j_iters_ = 0;
     float gx_ = 0.0f, gy_ = 0.0f, h_ = 0.0f;
     bool collisions_ = true, damped_ = false, have_forward_ = false;
     std::vector<uint8_t> nonfinite_;  // per world, after the last backward: a gradient entry is inf or NaN
@@ -126,6 +141,10 @@ private:
            *d_final_ = nullptr, *d_tape_s_ = nullptr, *d_lam_p_ = nullptr, *d_lam_q_ = nullptr, *d_dl_ = nullptr,
            *d_gv_ = nullptr;
     uint16_t* d_tape_nbr_ = nullptr;
+    float2 *d_tape_v_ = nullptr, *d_tape_pj_ = nullptr, *d_j_axis_ = nullptr, *d_lam_v_ = nullptr;
+    uint32_t *d_jc_start_ = nullptr, *d_j_ab_ = nullptr;
+    float4 *d_j_anchor_ = nullptr, *d_j_mass_ = nullptr;
+    float* d_j_dist_ = nullptr;
 };
 
 }  // namespace bg

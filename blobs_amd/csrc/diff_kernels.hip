@@ -16,8 +16,15 @@
 // constant).  With w_i = m_j / (m_i + m_j) and a = s_i - s_j, the
 // correction on i is w_i f(a), f(a) = (r_i + r_j - |a|) a / |a|, whose
 // Jacobian is J = (rs/|a|)(I - n n^T) - I; the spring force
-// F = u k (L - rest) has dF/dd = k[(1 - rest/L)(I - u u^T) + u u^T] and
-// dF/dk = u (L - rest).  Damped springs are refused by the host.
+// F = u k (L - rest) - c (rv . u) (u_x^2, u_y^2) (springs.rs:38-43, the
+// damping vector multiplies the direction componentwise), rv = v_a - v_b,
+// has dF/dd = k[(1 - rest/L)(I - u u^T) + u u^T] + (damping through u,
+// du/dd = (I - u u^T)/L), dF/drv = -c (u_x^2, u_y^2) u^T and
+// dF/dk = u (L - rest).  The velocity v = displacement / h of the previous
+// Verlet is part of the adjoint state with damping.  A fixed joint moves
+// p_a += w_a corr and p_b -= w_b corr, corr = (|d| - D) d/|d| with d the
+// anchor axis; dcorr/dd = I - (D/|d|)(I - n n^T), and the reverse sweep
+// walks the joints in the opposite order on the axes of the forward sweep.
 #include "diff.hpp"
 #include "step_common.hpp"
 
@@ -211,11 +218,75 @@ __device__ __forceinline__ void grid_query(const Ctx& c, float qx, float qy, flo
     }
 }
 
-// One forward substep of the world; records the snapshot and contact
-// partners at tape slot `k` when tape_s is non-null.
-__device__ void substep_forward(Ctx& c, Own& o, const DiffArgs& a, const DiffWorld& W, uint32_t gsub,
-                                float2* tape_s, uint16_t* tape_nbr, uint32_t k) {
+// Joint weights (physics.rs:450-460): a static -> b moves by ims corr; b
+// static -> a does; otherwise a by ratio corr and b by (1 - ratio) corr.
+__device__ __forceinline__ void joint_weights(const DiffArgs& a, const DiffWorld& W, uint32_t q, uint32_t la,
+                                              uint32_t lb, bool& sa, bool& sb, float4& jm) {
+    sa = (a.flags[W.b0 + la] & BF_STATIC) != 0;
+    sb = (a.flags[W.b0 + lb] & BF_STATIC) != 0;
+    jm = a.j_mass[q];
+}
+
+// Position sweep of one joint component (physics.rs:426-460) on the LDS
+// positions P, joint_iterations times in arena order; with `axis`, every
+// solve's anchor axis goes to axis[it * n_joints + q].
+__device__ void joint_sweep(float2* P, uint32_t comp, const DiffArgs& a, const DiffWorld& W, float2* axis) {
+    const uint32_t q0 = a.jc_start[comp], q1 = a.jc_start[comp + 1];
+    uint32_t flags = 0;
+    for (uint32_t it = 0; it < a.j_iters; ++it)
+        for (uint32_t q = q0; q < q1; ++q) {
+            const uint32_t ab = a.j_ab[q], la = ab & 0xFFFFu, lb = ab >> 16;
+            bool sa, sb;
+            float4 jm;
+            joint_weights(a, W, q, la, lb, sa, sb, jm);
+            const float4 an = a.j_anchor[q];
+            float2 pa = P[la], pb = P[lb];
+            float dx, dy;
+            const bool moved = joint_positions(pa, pb, make_float2(an.x, an.y), make_float2(an.z, an.w), a.j_dist[q],
+                                               jm, sa, sb, dx, dy, flags);
+            if (axis) axis[(size_t)it * a.n_joints + q] = make_float2(dx, dy);
+            if (!moved) continue;
+            P[la] = pa;
+            P[lb] = pb;
+        }
+}
+
+// Adjoint of joint_sweep on the LDS position adjoints L, from the axes the
+// forward sweep recorded: last solve first.
+__device__ void joint_sweep_reverse(float2* L, uint32_t comp, const DiffArgs& a, const DiffWorld& W,
+                                    const float2* axis) {
+    const uint32_t q0 = a.jc_start[comp], q1 = a.jc_start[comp + 1];
+    for (uint32_t it = a.j_iters; it-- > 0;)
+        for (uint32_t q = q1; q-- > q0;) {
+            const float2 d = axis[(size_t)it * a.n_joints + q];
+            const float dist = vlen(d.x, d.y);
+            if (dist < 1e-6f) continue;  // the reference skips the joint
+            const uint32_t ab = a.j_ab[q], la = ab & 0xFFFFu, lb = ab >> 16;
+            bool sa, sb;
+            float4 jm;
+            joint_weights(a, W, q, la, lb, sa, sb, jm);
+            const float wa = sa ? 0.0f : (sb ? jm.x : jm.y);
+            const float wb = sa ? jm.x : (sb ? 0.0f : jm.z);
+            const float2 la2 = L[la], lb2 = L[lb];
+            const float lcx = wa * la2.x - wb * lb2.x, lcy = wa * la2.y - wb * lb2.y;
+            const float nx = d.x / dist, ny = d.y / dist;
+            const float nm = nx * lcx + ny * lcy;
+            const float f = a.j_dist[q] / dist;
+            const float ldx = lcx - f * (lcx - nx * nm), ldy = lcy - f * (lcy - ny * nm);
+            L[la] = make_float2(la2.x - ldx, la2.y - ldy);
+            L[lb] = make_float2(lb2.x + ldx, lb2.y + ldy);
+        }
+}
+
+// One forward substep of the world; with `rec`, records the snapshot,
+// contact partners, velocities (damped springs) and joint-stage positions
+// (joints) at tape slot `k`.
+__device__ void substep_forward(Ctx& c, Own& o, const DiffArgs& a, const DiffWorld& W, uint32_t gsub, bool rec,
+                                uint32_t k) {
     const float h = a.h;
+    float2* tape_s = rec ? a.tape_s : nullptr;
+    uint16_t* tape_nbr = rec ? a.tape_nbr : nullptr;
+    float2* tape_v = rec ? a.tape_v : nullptr;
     if (a.collisions) build_grid(c, o, W);
     float2 pnew[kDiffPer], acc[kDiffPer];
 #pragma unroll
@@ -226,6 +297,7 @@ __device__ void substep_forward(Ctx& c, Own& o, const DiffArgs& a, const DiffWor
         const uint32_t li = o.li[j], gi = o.gi[j];
         const float4 own = c.snap[li];
         if (tape_s) tape_s[(size_t)k * a.nb_total + gi] = make_float2(own.x, own.y);
+        if (tape_v) tape_v[(size_t)k * a.nb_total + gi] = c.vel[li];
         // gravity + springs (physics.rs:369-375, springs.rs:25-47), ascending spring index
         if (!(o.fl[j] & BF_STATIC)) {
             float2 ac = gsub == 0 ? a.acc0[gi] : make_float2(0.0f, 0.0f);
@@ -352,6 +424,24 @@ __device__ void substep_forward(Ctx& c, Own& o, const DiffArgs& a, const DiffWor
             for (uint32_t t = 0; t + 1 < kDiffTapeNbr; ++t) r[t + 1] = rec[t];
         }
     }
+    // fixed joints, positions (physics.rs:424-460): the LDS velocity array is free until
+    // publish, so it holds the positions while the component lanes sweep
+    if (a.j_iters > 0 && W.njc > 0) {
+        __syncthreads();  // every spring has read the velocities
+#pragma unroll
+        for (uint32_t j = 0; j < kDiffPer; ++j)
+            if (o.live[j]) {
+                c.vel[o.li[j]] = pnew[j];
+                if (rec) a.tape_pj[(size_t)k * a.nb_total + o.gi[j]] = pnew[j];
+            }
+        __syncthreads();
+        for (uint32_t comp = W.jc0 + threadIdx.x; comp < W.jc0 + W.njc; comp += kDiffThreads)
+            joint_sweep(c.vel, comp, a, W, nullptr);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kDiffPer; ++j)
+            if (o.live[j]) pnew[j] = c.vel[o.li[j]];
+    }
     // update_objects (physics.rs:326-358); the snapshot is the new position
 #pragma unroll
     for (uint32_t j = 0; j < kDiffPer; ++j) {
@@ -380,32 +470,39 @@ __device__ void substep_forward(Ctx& c, Own& o, const DiffArgs& a, const DiffWor
     __syncthreads();
 }
 
-// Adjoint of substep `gsub` (tape slot k).  In: lp, lq = d loss / d (position,
-// position_old) after the substep.  Out: the same before it; gv += d loss /
-// d velocity_request at the substep that consumes it.
+// Adjoint of substep `gsub` (tape slot k).  In: lp, lq, lv = d loss / d
+// (position, position_old, calculated_velocity) after the substep.  Out: the
+// same before it; gv += d loss / d velocity_request at the substep that
+// consumes it.
 __device__ void substep_reverse(Ctx& c, const Own& o, const DiffArgs& a, const DiffWorld& W, uint32_t gsub,
-                                uint32_t k, float2* lp, float2* lq, float2* gv) {
+                                uint32_t k, float2* lp, float2* lq, float2* lv, float2* gv) {
     const float h = a.h;
+    const bool joints = a.j_iters > 0 && W.njc > 0;
     float2 lpp[kDiffPer], lqin[kDiffPer];
 #pragma unroll
     for (uint32_t j = 0; j < kDiffPer; ++j) {
         lpp[j] = make_float2(0.0f, 0.0f);
         lqin[j] = make_float2(0.0f, 0.0f);
         if (!o.live[j]) continue;
-        const float2 s = a.tape_s[(size_t)k * a.nb_total + o.gi[j]];
+        const size_t t = (size_t)k * a.nb_total + o.gi[j];
+        const float2 s = a.tape_s[t];
         c.snap[o.li[j]] = make_float4(s.x, s.y, o.r[j], o.m[j]);
+        if (joints) c.vel[o.li[j]] = a.tape_pj[t];  // positions entering the joint stage
+        else if (a.tape_v) c.vel[o.li[j]] = a.tape_v[t];
         float2 lam_a = make_float2(0.0f, 0.0f);
-        if (o.fl[j] & BF_STATIC) {  // p'' = p', q'' = p'
+        if (o.fl[j] & BF_STATIC) {  // p'' = p', q'' = p', v'' = 0
             lpp[j] = make_float2(lp[j].x + lq[j].x, lp[j].y + lq[j].y);
         } else {
+            // dx = (p' - q) r; p'' = p' + dx + a h^2, q'' = p', v'' = dx / h
             const float ratio = ((int32_t)o.li[j] == W.first_dyn && gsub == 0) ? h / W.old_dt0 : h / h;
-            if (gsub == 0 && (o.fl[j] & BF_HAS_VREQ)) {  // q = p' - vreq h: p'' = p' + vreq h r + a h^2
+            const float ldx = lp[j].x + lv[j].x / h, ldy = lp[j].y + lv[j].y / h;
+            if (gsub == 0 && (o.fl[j] & BF_HAS_VREQ)) {  // q = p' - vreq h: dx = vreq h r
                 lpp[j] = make_float2(lp[j].x + lq[j].x, lp[j].y + lq[j].y);
-                gv[j].x += h * ratio * lp[j].x;
-                gv[j].y += h * ratio * lp[j].y;
-            } else {  // p'' = p' + (p' - q) r + a h^2, q'' = p'
-                lpp[j] = make_float2((1.0f + ratio) * lp[j].x + lq[j].x, (1.0f + ratio) * lp[j].y + lq[j].y);
-                lqin[j] = make_float2(-ratio * lp[j].x, -ratio * lp[j].y);
+                gv[j].x += h * ratio * ldx;
+                gv[j].y += h * ratio * ldy;
+            } else {
+                lpp[j] = make_float2((lp[j].x + lq[j].x) + ratio * ldx, (lp[j].y + lq[j].y) + ratio * ldy);
+                lqin[j] = make_float2(-ratio * ldx, -ratio * ldy);
             }
             lam_a = make_float2(h * h * lp[j].x / o.m[j], h * h * lp[j].y / o.m[j]);
         }
@@ -413,8 +510,26 @@ __device__ void substep_reverse(Ctx& c, const Own& o, const DiffArgs& a, const D
         c.lam_am[o.li[j]] = lam_a;
     }
     __syncthreads();
+    if (joints) {
+        // replay each component's sweep from the recorded positions for its axes, then walk
+        // it backwards on the position adjoints
+        for (uint32_t comp = W.jc0 + threadIdx.x; comp < W.jc0 + W.njc; comp += kDiffThreads) {
+            joint_sweep(c.vel, comp, a, W, a.j_axis);
+            joint_sweep_reverse(c.lam_pp, comp, a, W, a.j_axis);
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kDiffPer; ++j)
+            if (o.live[j]) {
+                lpp[j] = c.lam_pp[o.li[j]];
+                if (a.tape_v) c.vel[o.li[j]] = a.tape_v[(size_t)k * a.nb_total + o.gi[j]];
+            }
+        __syncthreads();
+    }
+    float2 lvin[kDiffPer];
 #pragma unroll
     for (uint32_t j = 0; j < kDiffPer; ++j) {
+        lvin[j] = make_float2(0.0f, 0.0f);
         if (!o.live[j]) continue;
         const uint32_t li = o.li[j], gi = o.gi[j];
         const float4 si = c.snap[li];
@@ -468,20 +583,42 @@ __device__ void substep_reverse(Ctx& c, const Own& o, const DiffArgs& a, const D
             const float uf = ux * lfx + uy * lfy;
             const float kk = a.sp_k[s], rest = a.sp_rest[s];
             const float t = kk * (1.0f - rest / L);
-            const float kx = t * (lfx - ux * uf) + kk * ux * uf;
-            const float ky = t * (lfy - uy * uf) + kk * uy * uf;
+            float kx = t * (lfx - ux * uf) + kk * ux * uf;
+            float ky = t * (lfy - uy * uf) + kk * uy * uf;
+            float rvx = 0.0f, rvy = 0.0f;  // d loss / d (v_a - v_b)
+            const float cc = a.sp_c[s];
+            if (cc != 0.0f) {
+                const float2 va = c.vel[ia], vb = c.vel[ib];
+                const float vx = va.x - vb.x, vy = va.y - vb.y;
+                const float dot = vx * ux + vy * uy;
+                const float wl = ux * ux * lfx + uy * uy * lfy;
+                // d loss / d u of -c dot (u_x^2, u_y^2), then through du/dd = (I - u u^T)/L
+                const float gux = -cc * (vx * wl + 2.0f * dot * ux * lfx);
+                const float guy = -cc * (vy * wl + 2.0f * dot * uy * lfy);
+                const float gu = ux * gux + uy * guy;
+                kx += (gux - ux * gu) / L;
+                ky += (guy - uy * gu) / L;
+                rvx = -cc * wl * ux;
+                rvy = -cc * wl * uy;
+            }
             if (li == ia) {
                 ls.x -= kx;
                 ls.y -= ky;
+                lvin[j].x += rvx;
+                lvin[j].y += rvy;
                 a.grad_k[s] += (L - rest) * uf;
             } else {
                 ls.x += kx;
                 ls.y += ky;
+                lvin[j].x -= rvx;
+                lvin[j].y -= rvy;
             }
         }
         lp[j] = make_float2(lpp[j].x + ls.x, lpp[j].y + ls.y);
         lq[j] = lqin[j];
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kDiffPer; ++j) lv[j] = lvin[j];
     __syncthreads();  // snapshot and adjoint vectors are rewritten by the next substep
 }
 
@@ -552,7 +689,7 @@ __global__ __launch_bounds__(kDiffThreads) void k_diff_forward(DiffArgs a, uint3
                 }
         }
         for (uint32_t sub = 0; sub < a.substeps; ++sub)
-            substep_forward(c, o, a, W, step * a.substeps + sub, nullptr, nullptr, 0);
+            substep_forward(c, o, a, W, step * a.substeps + sub, false, 0);
     }
 #pragma unroll
     for (uint32_t j = 0; j < kDiffPer; ++j)
@@ -580,8 +717,11 @@ __global__ __launch_bounds__(kDiffThreads) void k_diff_backward(DiffArgs a, uint
     // recompute the segment from its checkpoint, recording the tape
     const uint32_t nsub = seg_steps * a.substeps;
     for (uint32_t k = 0; k < nsub; ++k)
-        substep_forward(c, o, a, W, first_step * a.substeps + k, a.tape_s, a.tape_nbr, k);
-    float2 lp[kDiffPer], lq[kDiffPer], gv[kDiffPer];
+        substep_forward(c, o, a, W, first_step * a.substeps + k, true, k);
+    float2 lp[kDiffPer], lq[kDiffPer], lv[kDiffPer], gv[kDiffPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kDiffPer; ++j)  // the loss does not read the final velocities
+        lv[j] = (o.live[j] && !last) ? a.lam_v[o.gi[j]] : make_float2(0.0f, 0.0f);
     if (last && !a.dl_dfinal) {  // d |centroid - target|^2 / d p_i = 2 (centroid - target) / n
         const double2 cen = block_centroid(c, o, W);
         const float gx = (float)(2.0 * (cen.x - (double)W.tx) / (double)(W.nb > 0 ? W.nb : 1));
@@ -600,12 +740,13 @@ __global__ __launch_bounds__(kDiffThreads) void k_diff_backward(DiffArgs a, uint
     }
 #pragma unroll
     for (uint32_t j = 0; j < kDiffPer; ++j) gv[j] = make_float2(0.0f, 0.0f);
-    for (uint32_t k = nsub; k-- > 0;) substep_reverse(c, o, a, W, first_step * a.substeps + k, k, lp, lq, gv);
+    for (uint32_t k = nsub; k-- > 0;) substep_reverse(c, o, a, W, first_step * a.substeps + k, k, lp, lq, lv, gv);
 #pragma unroll
     for (uint32_t j = 0; j < kDiffPer; ++j) {
         if (!o.live[j]) continue;
         a.lam_p[o.gi[j]] = lp[j];
         a.lam_q[o.gi[j]] = lq[j];
+        a.lam_v[o.gi[j]] = lv[j];
         if (seg == 0) a.grad_vreq[o.gi[j]] = gv[j];
     }
 }

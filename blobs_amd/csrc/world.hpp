@@ -41,8 +41,11 @@ struct DiffHostWorld {
     std::vector<uint32_t> flags;
     std::vector<int32_t> sa, sb;  // spring endpoints, dense body index
     std::vector<float> rest, k, c;
+    std::vector<int32_t> ja, jb;  // fixed joints in arena order, dense body index
+    std::vector<float2> jan_a, jan_b;
+    std::vector<float> jdist;
     float old_dt = 1.0f, gx = 0.0f, gy = 0.0f;
-    uint32_t substeps = 8;
+    uint32_t substeps = 8, joint_iterations = 4;
     bool collisions = true;
 };
 

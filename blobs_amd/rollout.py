@@ -1,7 +1,7 @@
 """Differentiable rollouts of batches of `Physics` worlds on the GPU.
 
 `Rollout(worlds, targets, checkpoint_every)` copies single-sphere worlds with
-springs into device buffers; `forward(delta, n_steps)` runs n_steps calls of
+springs (damped or not) and fixed joints into device buffers; `forward(delta, n_steps)` runs n_steps calls of
 Physics::step(delta) (physics.rs:78-82) keeping a checkpoint every
 `checkpoint_every` steps, and `backward()` returns the gradients of the
 summed centroid loss |centroid(final positions) - target|^2 with respect to

@@ -309,14 +309,19 @@ def uneven_batch_world(size, packing, seed_rng: Rng, vel_sigma=1.0):
                                 {"config": "batched_world", "box_side": side, "forced_overlaps": int(forced)})
 
 
-def rollout_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0, kmax=1e3):
+def rollout_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0, kmax=1e3, damping=0.0,
+                  n_joints=0, anchor=0.0):
     """One differentiable-rollout world: n single spheres r~U[0.5,1.0] at the
     given packing (square box centred on the origin: position Verlet forms
     velocities as p - p_old in f32, so small coordinates keep them
     accurate), velocity requests N(0, vel_sigma),
     n_springs springs between random bodies within distance 6 (rest =
-    initial distance, stiffness logU[kmin, kmax], damping 0), gravity on,
-    dt = 1/120."""
+    initial distance, stiffness logU[kmin, kmax], damping 0 or, with
+    damping > 0, U[0, damping]), gravity on, dt = 1/120.  n_joints > 0 adds
+    fixed joints between random bodies within distance 6, anchors
+    U[-anchor, anchor]^2 (distance and target angle from create_fixed_joint).
+    The extra draws come after the default ones, so the default world is
+    unchanged."""
     side = box_side_for_packing(int(n), 0.5, 1.0, float(packing))
     pos = np.zeros((n, 2), np.float32)
     vel = np.zeros((n, 2), np.float32)
@@ -335,6 +340,18 @@ def rollout_world(n, n_springs, rng: Rng, packing=0.4, vel_sigma=2.0, kmin=10.0,
                                "spring_damping": 0.0})
     sc.spring_a, sc.spring_b, sc.spring_rest = sa, sb, rest
     sc.spring_k, sc.spring_c = k, np.zeros(n_springs, np.float32)
+    if damping > 0.0:
+        sc.spring_c = rng.uniform(n_springs, 0.0, damping).astype(np.float32)
+        sc.meta["spring_damping"] = float(damping)
+    if n_joints > 0:
+        ja = np.zeros(n_joints, np.int32)
+        jb = np.zeros(n_joints, np.int32)
+        jr = np.zeros(n_joints, np.float32)
+        made = _scenes_lib().sg_pairs(rng.st, int(n), _f(pos), int(n_joints), 6.0, _i(ja), _i(jb), _f(jr))
+        assert made == n_joints
+        sc.joint_a, sc.joint_b = ja, jb
+        sc.joint_anchor_a = rng.uniform(2 * n_joints, -anchor, anchor).reshape(n_joints, 2).astype(np.float32)
+        sc.joint_anchor_b = rng.uniform(2 * n_joints, -anchor, anchor).reshape(n_joints, 2).astype(np.float32)
     return sc
 
 

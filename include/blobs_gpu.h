@@ -301,17 +301,20 @@ int bg_autotune(bg_world *w, double delta, uint32_t steps_per_trial, float *cell
                 uint32_t *rebuild_interval);
 
 /* ---- differentiable rollouts (GPU-native; no counterpart in the reference)
- * A rollout batches single-sphere worlds with springs (one centred,
- * non-sensor collider per body inserted in body order; no joints or circle
- * constraints; at most 2,048 bodies per world; worlds share gravity,
- * substeps and collisions_enabled) and runs n_steps calls of
- * Physics::step(delta) (physics.rs:78-82) on copies of their state, keeping
- * a checkpoint every `checkpoint_every` steps.  Parameters: the initial
- * velocity requests (one Vec2 per body, rigid_body.rs:182-184) and the
- * spring stiffnesses (springs.rs:16-22).  The loss of world w is
- * |centroid(final positions) - target_w|^2.  The adjoint differentiates
- * the step with every substep's contact pair set held fixed, recomputing
- * each segment from its checkpoint; damped springs are refused by it. */
+ * A rollout batches single-sphere worlds with springs (damped or not) and
+ * fixed joints (one centred, non-sensor collider per body inserted in body
+ * order; no circle constraints; at most 2,048 bodies per world; worlds
+ * share gravity, substeps, joint_iterations and collisions_enabled) and
+ * runs n_steps calls of Physics::step(delta) (physics.rs:78-82) on copies
+ * of their state, keeping a checkpoint every `checkpoint_every` steps.
+ * Parameters: the initial velocity requests (one Vec2 per body,
+ * rigid_body.rs:182-184) and the spring stiffnesses (springs.rs:16-22).
+ * The loss of world w is |centroid(final positions) - target_w|^2.  The
+ * adjoint differentiates every stage of the step (gravity, springs with
+ * their damping, contacts, joint positions, Verlet) with every substep's
+ * contact pair set held fixed, recomputing each segment from its
+ * checkpoint.  Body rotations are not tracked: with centred colliders and
+ * unrotated joint anchors (physics.rs:434-435) no position depends on them. */
 typedef struct bg_rollout bg_rollout;
 int bg_rollout_create(bg_world *const *worlds, uint32_t n_worlds, const float *targets_xy, uint32_t checkpoint_every,
                       bg_rollout **out);

@@ -7,9 +7,13 @@ A restatement of blobs' substep for single-sphere worlds with springs, in
 float64 torch ops so autograd gives the exact gradient of the same map the
 GPU adjoint differentiates:
 
-  gravity + springs  physics.rs:369-375, springs.rs:25-47 (F = u (k(L-rest) - D))
+  gravity + springs  physics.rs:369-375, springs.rs:25-47 (F = u (k(L-rest) - D),
+                     D = c ((v_a - v_b) . u) u, v = calculated_velocity)
   contacts           physics.rs:241-317 (snapshot = positions at the substep
                      start; A = the higher collider index; rho = 1 - mA/(mA+mB))
+  fixed joints       physics.rs:424-477, positions only (anchors are not
+                     rotated, so rotations never reach a position): Gauss-
+                     Seidel in arena order, joint_iterations sweeps
   update_objects     physics.rs:326-358 (velocity request consumed once,
                      Q1 ratio h/old_dt for the first dynamic body)
 
@@ -28,7 +32,7 @@ import torch
 class RefWorlds:
     """Concatenated worlds (arrays from blobs_amd.scenes.Scene objects)."""
 
-    def __init__(self, scenes, targets):
+    def __init__(self, scenes, targets, joint_iterations=4):
         n = [s.n_bodies for s in scenes]
         self.b0 = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
         cat = lambda f: np.concatenate([f(s) for s in scenes])
@@ -49,6 +53,20 @@ class RefWorlds:
         self.k0 = torch.tensor(cat(lambda s: s.spring_k), dtype=torch.float64)
         self.c = torch.tensor(cat(lambda s: s.spring_c), dtype=torch.float64)
         self.gravity = torch.tensor(scenes[0].gravity, dtype=torch.float64)
+        # fixed joints, arena (= creation) order per world; distance as create_fixed_joint computes it in f32
+        from .oracle import joint_setup
+        ja, jb, an_a, an_b, jd = [], [], [], [], []
+        for s, o in zip(scenes, self.b0[:-1]):
+            ja.append(np.asarray(s.joint_a, np.int64) + o)
+            jb.append(np.asarray(s.joint_b, np.int64) + o)
+            an_a.append(np.asarray(s.joint_anchor_a, np.float64).reshape(-1, 2))
+            an_b.append(np.asarray(s.joint_anchor_b, np.float64).reshape(-1, 2))
+            jd.append(joint_setup(s)[0].astype(np.float64) if len(s.joint_a) else np.zeros(0))
+        self.ja, self.jb = np.concatenate(ja), np.concatenate(jb)
+        self.jan_a = torch.tensor(np.concatenate(an_a))
+        self.jan_b = torch.tensor(np.concatenate(an_b))
+        self.jdist = torch.tensor(np.concatenate(jd))
+        self.joint_iterations = joint_iterations
         self.targets = torch.tensor(np.asarray(targets, np.float64).reshape(len(scenes), 2))
         self.n_worlds = len(scenes)
         dyn = (~self.static).numpy()
@@ -111,6 +129,8 @@ class RefWorlds:
                     corr = corr.index_add(0, A, rho * delta_ * n)
                     corr = corr.index_add(0, B, -(1.0 - rho) * delta_ * n)
                     p1 = p + corr
+            if self.ja.shape[0] and self.joint_iterations > 0:
+                p1 = self._joints(p1)
             if gsub == 0:
                 q = torch.where(self.has_vreq[:, None], p1 - vreq * h, q)
                 ratio = (h / old_dt0) * first[:, None] + (1.0 - first[:, None])
@@ -121,6 +141,44 @@ class RefWorlds:
             p = dyn * pn + (1.0 - dyn) * p1
             q = p1
             v = dyn * disp / h
+        return p
+
+    def _joint_levels(self):
+        """Joints in levels: a joint's level is one more than that of the
+        latest earlier joint sharing a body, so the joints of one level touch
+        disjoint bodies and every joint they depend on sits in a lower level;
+        running the levels in order is the arena-order sweep."""
+        last = {}
+        lev = np.zeros(self.ja.shape[0], np.int64)
+        for q in range(self.ja.shape[0]):
+            a, b = int(self.ja[q]), int(self.jb[q])
+            lev[q] = max(last.get(a, -1), last.get(b, -1)) + 1
+            last[a] = last[b] = lev[q]
+        return [np.nonzero(lev == k)[0] for k in range(int(lev.max()) + 1)] if lev.size else []
+
+    def _joints(self, p):
+        """Fixed joints, positions (physics.rs:434-460): arena order,
+        joint_iterations sweeps; a joint whose anchors coincide (distance <
+        1e-6, decided on the value) is skipped.  Weights: a static -> b moves
+        by (1/m_a + 1/m_b) corr; b static -> a does; else a moves by
+        ratio corr and b by (1 - ratio) corr, ratio = (1/m_a) / (1/m_a + 1/m_b)."""
+        if not hasattr(self, "_levels"):
+            self._levels = self._joint_levels()
+        m = self.mass
+        stat = self.static
+        for _ in range(self.joint_iterations):
+            for idx in self._levels:
+                a, b = torch.as_tensor(self.ja[idx]), torch.as_tensor(self.jb[idx])
+                d = (p[b] + self.jan_b[idx]) - (p[a] + self.jan_a[idx])
+                dist = d.norm(dim=1, keepdim=True)
+                keep = (dist.detach() >= 1e-6).to(torch.float64)
+                corr = keep * (dist - self.jdist[idx][:, None]) * d / torch.where(keep > 0, dist, torch.ones_like(dist))
+                ims = 1.0 / m[a] + 1.0 / m[b]
+                ratio = (1.0 / m[a]) / ims
+                sa, sb = stat[a], stat[b]
+                wa = torch.where(sa, torch.zeros_like(ims), torch.where(sb, ims, ratio))[:, None]
+                wb = torch.where(sa, ims, torch.where(sb, torch.zeros_like(ims), 1.0 - ratio))[:, None]
+                p = p.index_add(0, a, wa * corr).index_add(0, b, -wb * corr)
         return p
 
     def pairs_f32(self, p32: np.ndarray) -> np.ndarray:
@@ -139,17 +197,21 @@ class RefWorlds:
             out.append(np.stack([ii + b, jj + b], 1))
         return np.concatenate(out) if out else np.zeros((0, 2), np.int64)
 
-    def tape_adjoint(self, tape, pairs, final_p, delta, substeps=8, dl_dfinal=None):
+    def tape_adjoint(self, tape, pairs, final_p, delta, substeps=8, dl_dfinal=None, dtype=np.float64):
         """FP64 reverse pass along a recorded trajectory: tape[g] = positions
         at the start of substep g (the contact snapshot), pairs[g] = its
         contact pairs.  Returns (d loss / d vreq, d loss / d k) of the
         centroid loss (or of sum(dl_dfinal . final positions)).  Written
-        from the substep map's derivative, independently of autograd."""
-        h = float(np.float32(np.float32(delta) / np.float32(substeps)))
-        m = self.mass.numpy()
-        r = self.radius.numpy()
+        from the substep map's derivative, independently of autograd.
+        dtype=np.float32 runs the same pass in single precision, which
+        measures how far any f32 reverse pass may stand from the FP64 one."""
+        if self.ja.shape[0] or bool((self.c != 0).any()):
+            raise NotImplementedError("tape_adjoint covers undamped springs and contacts; use gradients()")
+        h = dtype(np.float32(np.float32(delta) / np.float32(substeps)))
+        m = self.mass.numpy().astype(dtype)
+        r = self.radius.numpy().astype(dtype)
         sa, sb = self.sa.numpy(), self.sb.numpy()
-        rest, k = self.rest.numpy(), self.k0.numpy()
+        rest, k = self.rest.numpy().astype(dtype), self.k0.numpy().astype(dtype)
         dyn = ~self.static.numpy()
         n = m.shape[0]
         fp = np.asarray(final_p, np.float64)
@@ -160,15 +222,16 @@ class RefWorlds:
                 lp[b:e] = 2.0 * (fp[b:e].mean(0) - self.targets[w].numpy()) / (e - b)
         else:
             lp = np.asarray(dl_dfinal, np.float64).copy()
-        lq = np.zeros((n, 2))
-        gv = np.zeros((n, 2))
-        gk = np.zeros(k.shape[0])
+        lp = lp.astype(dtype)
+        lq = np.zeros((n, 2), dtype)
+        gv = np.zeros((n, 2), dtype)
+        gk = np.zeros(k.shape[0], dtype)
         first = np.zeros(n, bool)
         first[[f for f in self.first_dyn if f >= 0]] = True
         has_v = self.has_vreq.numpy()
         for g in range(len(tape) - 1, -1, -1):
-            s = np.asarray(tape[g], np.float64)
-            ratio = np.where(first & (g == 0), h / 1.0, 1.0)[:, None]
+            s = np.asarray(tape[g], dtype)
+            ratio = np.where(first & (g == 0), h / 1.0, 1.0).astype(dtype)[:, None]
             if g == 0:
                 cons = (dyn & has_v)[:, None]
                 lpp = np.where(cons, lp + lq, (1.0 + ratio) * lp + lq)
@@ -180,7 +243,7 @@ class RefWorlds:
             lpp = np.where(dyn[:, None], lpp, lp + lq)
             lqin = np.where(dyn[:, None], lqin, 0.0)
             lam_am = np.where(dyn[:, None], h * h * lp / m[:, None], 0.0)
-            ls = np.zeros((n, 2))
+            ls = np.zeros((n, 2), dtype)
             pr = np.asarray(pairs[g]).reshape(-1, 2)
             if pr.shape[0]:
                 for i, j in ((pr[:, 0], pr[:, 1]), (pr[:, 1], pr[:, 0])):

@@ -183,11 +183,15 @@ def test_differentiable_rollout_full_shape_forward_bitexact_on_sampled_worlds():
 def test_differentiable_rollout_full_shape_gradients_on_sampled_worlds():
     """Gradients of the full rollout (256 worlds, checkpoint every 20) after 50
     and 200 steps against the FP64 reverse pass along the same f32 trajectory
-    (fixture: tests/golden/rollout_grads.npz).  Worlds 0 and 1: within 1e-3
-    relative (vector norm) for velocity requests and stiffnesses.  Worlds 3
-    and 97: their FP64 gradients exceed the f32 range at 200 steps (the
-    fixed-pair derivative of the contact projection grows about 1e8-fold every
-    50 steps), so the rollout reports them in nonfinite_worlds."""
+    (fixture: tests/golden/rollout_grads.npz).  Worlds 0 and 1: relative
+    error (vector norm) for velocity requests and stiffnesses within 1e-3, or
+    within twice the error of the same reverse pass run in f32 on the CPU
+    where that is larger (world 0 after 50 steps: 2.3e-3 and 3.4e-3 -- the
+    conditioning of the trajectory, not the summation order, sets it).
+    Worlds 3 and 97: their FP64 gradients exceed the f32 range at 200 steps
+    (the fixed-pair derivative of the contact projection grows about
+    1e8-fold every 50 steps), so the rollout reports them in
+    nonfinite_worlds."""
     from blobs_amd.rollout import Rollout
     ref = np.load(GOLDEN / "rollout_grads.npz")
     ws, targets = scenes.rollout_scenes()
@@ -203,7 +207,10 @@ def test_differentiable_rollout_full_shape_gradients_on_sampled_worlds():
         for w in (0, 1):
             ev = rel(gv[2048 * w:2048 * (w + 1)], ref[f"w{w}_h{horizon}_gv"])
             ek = rel(gk[1024 * w:1024 * (w + 1)], ref[f"w{w}_h{horizon}_gk"])
-            assert ev <= 1e-3 and ek <= 1e-3, f"world {w}, {horizon} steps: rel error {ev:.2e} (v), {ek:.2e} (k)"
+            bv = max(1e-3, 2.0 * float(ref[f"w{w}_h{horizon}_f32rel_v"]))
+            bk = max(1e-3, 2.0 * float(ref[f"w{w}_h{horizon}_f32rel_k"]))
+            assert ev <= bv and ek <= bk, (f"world {w}, {horizon} steps: rel error {ev:.2e} (v, bar {bv:.2e}), "
+                                           f"{ek:.2e} (k, bar {bk:.2e})")
         bad = set(r.nonfinite_worlds.tolist())
         assert not bad & {0, 1}
         if horizon == 200:
