@@ -112,11 +112,15 @@ def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) 
         cs, sk, iv = batch.autotune(dt, 4)
         tuned = {"cell_scale": cs, "skin": sk, "rebuild_interval": iv}
         log(f"batched: autotune -> {tuned} ({time.perf_counter() - t0:.1f} s)")
-    batch.run_chunks(dt, chunk_steps, 1)  # graph capture warm-up
-    batch._lib.bg_synchronize(batch._w)
-    t0 = time.perf_counter()
-    batch.run_chunks(dt, chunk_steps, n_chunks)  # synchronizes at the end
-    elapsed = time.perf_counter() - t0
+    def timed(n):
+        batch.run_chunks(dt, chunk_steps, 1)  # graph capture warm-up
+        batch._lib.bg_synchronize(batch._w)
+        t0 = time.perf_counter()
+        batch.run_chunks(dt, chunk_steps, n)  # synchronizes at the end
+        return time.perf_counter() - t0
+
+    elapsed = timed(n_chunks)
+    persistent = batch.persistent_active()
     steps = chunk_steps * n_chunks
     if not np.isfinite(batch.bodies()["position"]).all():
         raise RuntimeError("non-finite positions in the batched run")
@@ -125,8 +129,18 @@ def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) 
         "worlds": n_worlds, "bodies": n_bodies, "steps": steps, "chunk_steps": chunk_steps,
         "value": n_bodies * steps / elapsed, "unit": "sphere-steps/s", "ms_per_step": 1000.0 * elapsed / steps,
         "autotuned": tuned,
+        "schedule": "persistent (one workgroup per world, one launch per chunk)" if persistent
+        else "grid path (one launch per substep)",
     }
-    log(f"batched: {steps} steps in {elapsed:.2f} s -> {rec['value']:.4g} sphere-steps/s")
+    log(f"batched: {steps} steps in {elapsed:.2f} s -> {rec['value']:.4g} sphere-steps/s ({rec['schedule']})")
+    if persistent:
+        # the grid path on the same batch, for comparison (same results, other schedule)
+        batch.set_persistent(False)
+        n_cmp = max(1, n_chunks // 4)
+        e2 = timed(n_cmp)
+        rec["grid_path_value"] = n_bodies * chunk_steps * n_cmp / e2
+        log(f"batched: grid path {chunk_steps * n_cmp} steps in {e2:.2f} s -> {rec['grid_path_value']:.4g} "
+            "sphere-steps/s")
     return rec
 
 

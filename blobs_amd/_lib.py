@@ -186,6 +186,8 @@ SIGNATURES = {
     "bg_batch_create": [C.POINTER(WORLD), C.c_uint32, C.POINTER(WORLD)],
     "bg_batch_writeback": [WORLD, C.POINTER(WORLD), C.c_uint32],
     "bg_batch_size": [WORLD, u32p],
+    "bg_batch_set_persistent": [WORLD, C.c_int],
+    "bg_batch_persistent": [WORLD, C.POINTER(C.c_int)],
     "bg_autotune": [WORLD, C.c_double, C.c_uint32, f32p, f32p, u32p],
     "bg_set_event_capacity": [WORLD, C.c_uint32],
     "bg_drain_events": [WORLD, C.POINTER(CollisionEventC), C.c_uint32, u32p, u64p],

@@ -24,7 +24,7 @@ ROOT = PKG.parent
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BLOBS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sort.hip", "step_kernels.hip", "sphere_kernels.hip", "ordered_contacts.hip", "spatial.hip", "diff_kernels.hip", "world.cpp", "batch.cpp",
+HIP_SOURCES = ["sort.hip", "step_kernels.hip", "sphere_kernels.hip", "ordered_contacts.hip", "spatial.hip", "diff_kernels.hip", "batch_kernels.hip", "world.cpp", "batch.cpp",
                "diff.cpp", "capi.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -155,7 +155,7 @@ namespace {
 // Host-side state a captured chunk depends on: the chunk may be replayed
 // only from the state it was captured in.
 struct LaunchState {
-    const void* p[6];
+    const void* p[8];
     uint32_t u[5];
     bool operator==(const LaunchState& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 };
@@ -176,6 +176,8 @@ int World::step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks) {
         s.p[3] = d.s_alt.pp;
         s.p[4] = d.b.pos;
         s.p[5] = d.keys;
+        s.p[6] = p_src_.pp;  // the persistent launch reads its worlds from here
+        s.p[7] = p_src_.snap;
         s.u[0] = bbox_parity_;
         s.u[1] = steps_since_rebuild_;
         s.u[2] = (need_rebuild_ ? 1u : 0u) | (d.gather_pending ? 2u : 0u) | (fresh_ ? 4u : 0u);
@@ -269,10 +271,27 @@ int World::step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks) {
     return check_and_process();
 }
 
+int World::set_persistent(bool enable) {
+    if (enable == persist_enabled_) return BG_OK;
+    if (sync_host()) return BG_ERR_HIP;  // the state moves through the host mirror
+    persist_enabled_ = enable;
+    mark_dirty();
+    return BG_OK;
+}
+
 int World::autotune(double delta, uint32_t steps_per_trial, float* cell_scale, float* skin, uint32_t* interval) {
     if (steps_per_trial == 0) steps_per_trial = 1;
     int st = prepare_step();
     if (st) return st;
+    // a batch the persistent step can run: the grid candidates are timed on the grid path
+    // alone, then the persistent schedule with the best of them (each switch re-uploads
+    // through the host mirror)
+    const bool try_persist = persist_;
+    if (try_persist) {
+        st = set_persistent(false);
+        if (st == BG_OK) st = prepare_step();
+        if (st) return st;
+    }
     struct Cand {
         float cs, sk;
         uint32_t iv;
@@ -321,6 +340,35 @@ int World::autotune(double delta, uint32_t steps_per_trial, float* cell_scale, f
     if (!have) return fail(BG_ERR_HIP, "autotune: event timing failed");
     st = set_broadphase(best.cs, best.sk, best.iv);
     if (st) return st;
+    if (try_persist) {
+        st = set_persistent(true);
+        if (st == BG_OK) st = prepare_step();
+        if (st) return st;
+        float persist_ms = -1.0f;
+        hipEvent_t p0 = nullptr, p1 = nullptr;
+        HIP_TRY(hipEventCreate(&p0));
+        HIP_TRY(hipEventCreate(&p1));
+        st = launch_steps(delta, 1);  // untimed: consumes the upload's one-off inputs, first grid
+        if (st == BG_OK) advance_time(delta, 1);
+        hipError_t e = hipSuccess;
+        if (st == BG_OK) e = hipEventRecord(p0, d_.stream);
+        if (st == BG_OK && e == hipSuccess) st = launch_steps(delta, steps_per_trial);
+        if (st == BG_OK && e == hipSuccess) {
+            advance_time(delta, steps_per_trial);
+            e = hipEventRecord(p1, d_.stream);
+            if (e == hipSuccess) e = hipEventSynchronize(p1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&persist_ms, p0, p1);
+        }
+        (void)hipEventDestroy(p0);
+        (void)hipEventDestroy(p1);
+        host_stale_ = true;
+        if (st) return st;
+        if (e != hipSuccess) return fail(BG_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        if (persist_ms > best_ms) {
+            st = set_persistent(false);
+            if (st) return st;
+        }
+    }
     if (cell_scale) *cell_scale = best.cs;
     if (skin) *skin = best.sk;
     if (interval) *interval = best.iv;

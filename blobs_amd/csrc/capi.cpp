@@ -211,6 +211,18 @@ int bg_batch_size(bg_world* w, uint32_t* out) {
     });
 }
 
+int bg_batch_set_persistent(bg_world* w, int enable) {
+    return guarded(w, [&](bg::World& x) { return x.set_persistent(enable != 0); });
+}
+
+int bg_batch_persistent(bg_world* w, int* active) {
+    if (!active) return BG_ERR_INVALID_ARGUMENT;
+    return guarded(w, [&](bg::World& x) {
+        *active = x.persistent_active() ? 1 : 0;
+        return BG_OK;
+    });
+}
+
 int bg_autotune(bg_world* w, double delta, uint32_t steps_per_trial, float* cell_scale, float* skin,
                 uint32_t* rebuild_interval) {
     return guarded(w, [&](bg::World& x) { return x.autotune(delta, steps_per_trial, cell_scale, skin, rebuild_interval); });

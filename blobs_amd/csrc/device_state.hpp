@@ -89,6 +89,21 @@ enum : uint32_t { OT_PUSH = 1u, OT_CORR = 2u };
 constexpr uint32_t kOrdCand = 8192;   // candidate colliders of one replayed row
 constexpr uint32_t kOrdTerms = 8192;  // addends of one body's position sum
 
+// Persistent batched step (batch_kernels.hip): the worlds of a batch with
+// kPersistMinBodies to kLdsBodies bodies, each run by ONE workgroup of
+// kPersistClassThreads threads for every substep of a launch, bodies in
+// registers (kPersistPer per thread), snapshot and grid in LDS.  Smaller
+// worlds are cheaper on the grid path: a world of a few hundred bodies is
+// too little parallel work for a workgroup's serial chain of grid-build
+// phases per substep (measured on the 4,096-world batch: 0.45 vs 0.20 ms per
+// step for its worlds up to 256 bodies, 0.59 vs 0.41 up to 1,024, 0.73 vs
+// 1.12 from 1,025 to 4,096).
+constexpr uint32_t kPersistPer = 4;
+constexpr uint32_t kPersistClasses = 1;
+constexpr uint32_t kPersistClassThreads[kPersistClasses] = {1024};
+constexpr uint32_t kLdsBodies = kPersistPer * 1024;
+constexpr uint32_t kPersistMinBodies = 1025;
+
 struct DeviceWorld {
     uint32_t nb = 0, nc = 0, ns = 0, nj = 0, nk = 0;
     uint32_t cells_cap = 0;
@@ -198,6 +213,18 @@ struct DeviceWorld {
     uint32_t n_block_slots = 0;
     SortTemp sort{};
 
+    // persistent batched step (batch_kernels.hip): the persistent worlds by size class
+    // (largest first in each), the queue counters, and per world whether the grid path
+    // skips it (it does when the persistent step runs the world)
+    uint32_t *p_order = nullptr, *p_wstart = nullptr, *p_counter = nullptr;
+    uint8_t *p_skip = nullptr;
+    uint32_t p_class_start[kPersistClasses + 1] = {};
+    uint32_t p_groups[kPersistClasses] = {};
+    uint32_t p_grid_worlds = 0;  // worlds left to the grid path
+    // the persistent launch runs on its own stream beside the grid path's substeps
+    // (default priority; forked from and joined to `stream` by events)
+    hipStream_t p_stream = nullptr;
+    hipEvent_t p_fork = nullptr, p_join = nullptr;
     // collision events (optional): buffer of event_cap records, count in accum
     DevEvent *events = nullptr;
     uint32_t event_cap = 0;
@@ -210,6 +237,28 @@ struct DeviceWorld {
     // at the kernel's own start and end (hipExtLaunchKernelGGL)
     hipEvent_t prof_start = nullptr, prof_stop = nullptr;
 };
+
+// Arguments of one persistent launch.
+struct PersistArgs {
+    // sphere-path records, slot = dense index (the grid path never moves the slots of a
+    // persistent world): read from `src`, where the previous persistent launch left them,
+    // written to `dst`, the records current after the grid path's substeps of this launch
+    SphereArrays src, dst;
+    const uint32_t* wstart;  // [n_worlds + 1] slot range of each world
+    float* wold_dt;          // [n_worlds] Physics::old_dt (Q1)
+    const uint32_t* order;   // persistent worlds by class, largest first within a class
+    uint32_t class_start[kPersistClasses + 1];  // class c: order[class_start[c], class_start[c+1])
+    uint32_t groups[kPersistClasses];           // workgroups launched per class
+    uint32_t* counter;                          // [kPersistClasses] queue positions
+    StepAccum* accum;
+    unsigned long long* block_contacts;
+    float h, ratio_rest, gx, gy;
+    uint32_t nsub;        // substeps of this launch
+    uint32_t fresh;       // its first substep consumes user accelerations and torques
+    uint32_t collisions;
+};
+
+hipError_t launch_batch_persistent(const PersistArgs& k, hipStream_t s);
 
 // Scalars of one substep launch.
 struct SubstepArgs {

@@ -86,6 +86,7 @@ struct SphereSubstep {
     uint32_t* nbr;      // partner lists (two 16-bit slot deltas a word): written by the first
                         // substep after a rebuild, read by the rest
     uint8_t* nbr_len;
+    const uint8_t* __restrict__ skip;  // batch: per world, the persistent step runs it (or null)
 };
 
 __device__ __forceinline__ int32_t free_cell(float x, float o, float inv) {
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
     // world of this slot (lanes past the end take the last body's world)
     const uint32_t w = BATCHED ? k.wid[s < k.nb ? s : k.nb - 1] : 0u;
     uint32_t new_outliers = 0, count = 0, pushes = 0, flags = 0;
-    if (s < k.nb) {
+    if (s < k.nb && !(BATCHED && k.skip && k.skip[w])) {
         const uint32_t g_own = GATHER ? k.perm[s] : s;
         const uint32_t fl = k.src.bflags[g_own];
         const float4 pp = k.src.pp[g_own];
@@ -637,13 +638,15 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
                                                           const float2* __restrict__ vel,
                                                           const uint32_t* __restrict__ bflags, uint32_t nb,
                                                           const uint32_t* __restrict__ wid, float* wstat,
-                                                          uint32_t rep, uint32_t* wbbox) {
+                                                          uint32_t rep, uint32_t* wbbox,
+                                                          const uint8_t* __restrict__ skip) {
     __shared__ float s_part[kThreads / BG_WAVE][8];
     __shared__ uint32_t s_bb[kThreads / BG_WAVE][4];
     __shared__ uint32_t s_w[kThreads / BG_WAVE];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < nb;
-    const uint32_t w = BATCHED ? wid[live ? i : nb - 1] : 0u;
+    const uint32_t w = BATCHED ? wid[i < nb ? i : nb - 1] : 0u;
+    // slots of worlds the persistent step runs are not the grid path's
+    const bool live = i < nb && !(BATCHED && skip && skip[w]);
     float v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = 0.0f;
@@ -810,35 +813,40 @@ __global__ __launch_bounds__(kThreads) void k_sphere_count(const float4* __restr
                                                            const GridParams* __restrict__ grids,
                                                            const uint32_t* __restrict__ cell_base,
                                                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
-                                                           uint32_t* __restrict__ rank) {
+                                                           uint32_t* __restrict__ rank,
+                                                           const uint8_t* __restrict__ skip) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < nb;
     uint32_t kk = kNil;
     if (live) {
         const uint32_t w = BATCHED ? wid[i] : 0u;
-        const GridParams g = grids[w];
-        const float4 sn = snap[i];
-        const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
-        const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
-        kk = cell_base[w] + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+        if (BATCHED && skip && skip[w]) {
+            // a world the persistent step runs: counted in its first cell so that the scan
+            // keeps its slot range (the scatter leaves its slots alone)
+            kk = cell_base[w];
+        } else {
+            const GridParams g = grids[w];
+            const float4 sn = snap[i];
+            const int32_t cx = cell_coord(sn.x, g.ox, g.inv_cell, g.nx);
+            const int32_t cy = cell_coord(sn.y, g.oy, g.inv_cell, g.ny);
+            kk = cell_base[w] + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+        }
         key[i] = kk;
     }
     // Slots are still in the previous rebuild's cell order and bodies move little, so
     // neighbouring lanes often share a cell: each run of equal keys takes its ranks with
     // one atomic (the run's first lane), the others offset from it.  Order inside a cell
-    // is arbitrary.
+    // is arbitrary.  A lane past the end carries kNil, which starts a run of its own.
     const uint32_t lane = threadIdx.x & (BG_WAVE - 1);
     const uint32_t prev = __shfl_up(kk, 1, BG_WAVE);
-    const bool head = live && (lane == 0u || prev != kk);
+    const bool head = lane == 0u || prev != kk;
     const uint64_t heads = __ballot(head);
     const uint64_t le = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);
     const uint32_t my_head = 63u - (uint32_t)__builtin_clzll((heads & le) | 1ull);  // this lane's run start
     const uint64_t above = heads & ~le;  // heads after this lane
-    const uint64_t live_mask = __ballot(live);
-    const uint32_t n_live = live_mask ? 64u - (uint32_t)__builtin_clzll(live_mask) : 0u;  // live lanes are a prefix
-    const uint32_t run_end = above ? (uint32_t)__builtin_ctzll(above) : n_live;
+    const uint32_t run_end = above ? (uint32_t)__builtin_ctzll(above) : 64u;
     uint32_t base = 0;
-    if (head) base = atomicAdd(&cnt[kk], run_end - lane);
+    if (head && live) base = atomicAdd(&cnt[kk], run_end - lane);
     base = __shfl(base, (int)my_head, BG_WAVE);
     if (live) rank[i] = base + (lane - my_head);
 }
@@ -861,6 +869,8 @@ struct SphereScatter {
     uint32_t nb;
     float* wstat;
     uint32_t n_wstat;
+    const uint32_t* __restrict__ wid;  // batch only
+    const uint8_t* __restrict__ skip;  // per world: the persistent step runs it (or null)
 };
 
 // Each snapshot to slot cell_start[key] + rank; the permutation records
@@ -878,6 +888,7 @@ __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
     }
     for (uint32_t h = i; h < 3u * k.n_waves; h += stride) k.waves[h / k.n_waves][h % k.n_waves] = 0u;
     if (i >= k.nb) return;
+    if (k.skip && k.skip[k.wid[i]]) return;
     const uint32_t slot = k.cell_start[k.key[i]] + k.rank[i];
     if (slot >= k.nb) return;  // consistent counts and scan never get here
     float4 sn = k.snap[i];
@@ -906,19 +917,19 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     const uint32_t blocks = div_up(d.nb, kThreads);
     if (d.n_worlds > 1)
         k_world_stats<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
-                                                               d.stat_rep, d.wbbox);
+                                                               d.stat_rep, d.wbbox, d.p_skip);
     else
         k_world_stats<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
-                                                                d.stat_rep, d.wbbox);
+                                                                d.stat_rep, d.wbbox, nullptr);
     k_world_grids<<<d.n_worlds, BG_WAVE, 0, d.stream>>>(d.n_worlds, d.wbbox, cell,
                                                                            d.cell_base, d.wstat, d.stat_rep,
                                                                            d.grids, d.wvel, d.accum);
     if (d.n_worlds > 1)
         k_sphere_count<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.wid, d.grids, d.cell_base,
-                                                                d.cell_count, d.keys, d.vals);
+                                                                d.cell_count, d.keys, d.vals, d.p_skip);
     else
         k_sphere_count<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.nb, d.wid, d.grids, d.cell_base,
-                                                                 d.cell_count, d.keys, d.vals);
+                                                                 d.cell_count, d.keys, d.vals, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = exclusive_scan_u32(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream);
@@ -943,6 +954,8 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     k.nb = d.nb;
     k.wstat = d.wstat;
     k.n_wstat = 8u * d.stat_rep * d.n_worlds;
+    k.wid = d.wid;
+    k.skip = d.n_worlds > 1 ? d.p_skip : nullptr;
     k_sphere_scatter<<<blocks, kThreads, 0, d.stream>>>(k);
     e = hipGetLastError();
     // the sorted snapshot becomes the current one; records follow on the next substep
@@ -998,6 +1011,7 @@ hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool grou
     k.vel_out = k.events ? d.vel_next : k.dst.vel;
     k.nbr = d.nbr;
     k.nbr_len = d.nbr_len;
+    k.skip = d.p_skip;
     const uint32_t blocks = div_up(d.nb, kSubstepThreads);
 #define BG_SPHERE(G, S, Z, C, P, B)                                                                        \
     do {                                                                                                  \

@@ -696,6 +696,9 @@ void World::free_arrays() {
     for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]), dfree(d.oh_wave[i]);
     dfree(d.cell_count), dfree(d.perm);
     dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt), dfree(d.wstat), dfree(d.wvel);
+    dfree(d.p_order), dfree(d.p_wstart), dfree(d.p_counter), dfree(d.p_skip);
+    for (uint32_t c = 0; c < kPersistClasses; ++c) d.p_groups[c] = 0;
+    d.p_grid_worlds = 0;
     dfree(d.snap_next), dfree(d.x0), dfree(d.bc_start), dfree(d.bc_list), dfree(d.nbr), dfree(d.nbr_len);
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
@@ -709,17 +712,26 @@ void World::free_arrays() {
     dfree(d.mv_list), dfree(d.ord_heap), dfree(d.ord_log), dfree(d.aff_list), dfree(d.aff_mark), dfree(d.ord_cand);
     dfree(d.ord_terms), dfree(d.clist), dfree(d.ccnt), dfree(d.cfb_list), dfree(d.cfb_n);
     sort_temp_free(d.sort);
-    hipStream_t s = d.stream;
+    hipStream_t s = d.stream, ps = d.p_stream;
+    hipEvent_t pf = d.p_fork, pj = d.p_join;
     int dev = d.device;
     d = DeviceWorld{};
     d.stream = s;
+    d.p_stream = ps;
+    d.p_fork = pf;
+    d.p_join = pj;
     d.device = dev;
 }
 
 void World::free_device() {
     free_arrays();
+    if (d_.p_stream) (void)hipStreamSynchronize(d_.p_stream);
+    if (d_.p_fork) (void)hipEventDestroy(d_.p_fork);
+    if (d_.p_join) (void)hipEventDestroy(d_.p_join);
+    if (d_.p_stream) (void)hipStreamDestroy(d_.p_stream);
     if (d_.stream) (void)hipStreamDestroy(d_.stream);
-    d_.stream = nullptr;
+    d_.stream = d_.p_stream = nullptr;
+    d_.p_fork = d_.p_join = nullptr;
 }
 
 // Whether two colliders of the snapshot pass the reference predicate with
@@ -1302,6 +1314,61 @@ int World::upload() {
     d.n_block_slots = (std::max<uint32_t>(std::max(nb, nc), 1) + 63) / 64;  // per-body and per-collider launches
     HIP_TRY(dmalloc(&d.block_contacts, d.n_block_slots));
     HIP_TRY(hipMemsetAsync(d.block_contacts, 0, sizeof(unsigned long long) * d.n_block_slots, d.stream));
+    // persistent batched step: a batch of centred single spheres with default groups, no
+    // sensors, constraints or events; its worlds of at most kLdsBodies bodies run there,
+    // the rest on the grid path
+    persist_ = false;
+    if (sphere && !batch_start_.empty() && persist_enabled_ && zero_off_ && !groups_ && !sensors_ && nk == 0 &&
+        event_cap_ == 0) {
+        const uint32_t W = batch_size();
+        std::vector<uint32_t> cls[kPersistClasses];
+        std::vector<uint8_t> skip(W, 0);
+        uint32_t grid_worlds = 0;
+        for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t n_w = batch_start_[w + 1] - batch_start_[w];
+            uint32_t c = kPersistClasses;
+            for (uint32_t q = 0; q < kPersistClasses; ++q)
+                if (n_w <= kPersistClassThreads[q] * kPersistPer) {
+                    c = q;
+                    break;
+                }
+            if (c == kPersistClasses || n_w < kPersistMinBodies) {
+                ++grid_worlds;
+                continue;
+            }
+            cls[c].push_back(w);
+            skip[w] = 1;
+        }
+        if (grid_worlds < W) {
+            // each class's queue: largest world first (longest processing time first)
+            std::vector<uint32_t> order;
+            int cus = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.device) != hipSuccess || cus < 1)
+                cus = 1;
+            // resident workgroups per CU of each class (LDS: about 105 KB a workgroup)
+            const uint32_t per_cu[kPersistClasses] = {1u};
+            d.p_class_start[0] = 0;
+            for (uint32_t c = 0; c < kPersistClasses; ++c) {
+                std::stable_sort(cls[c].begin(), cls[c].end(), [&](uint32_t a, uint32_t b) {
+                    return batch_start_[a + 1] - batch_start_[a] > batch_start_[b + 1] - batch_start_[b];
+                });
+                order.insert(order.end(), cls[c].begin(), cls[c].end());
+                d.p_class_start[c + 1] = (uint32_t)order.size();
+                d.p_groups[c] = std::min<uint32_t>((uint32_t)cls[c].size(), per_cu[c] * (uint32_t)cus);
+            }
+            d.p_grid_worlds = grid_worlds;
+            HIP_TRY(dmalloc(&d.p_order, order.size()));
+            HIP_TRY(dmalloc(&d.p_wstart, W + 1));
+            HIP_TRY(dmalloc(&d.p_counter, kPersistClasses));
+            HIP_TRY(dmalloc(&d.p_skip, W));
+            HIP_TRY(h2d(d.p_order, order, s));
+            HIP_TRY(h2d(d.p_wstart, batch_start_, s));
+            HIP_TRY(h2d(d.p_skip, skip, s));
+            HIP_TRY(hipStreamSynchronize(s));  // the host vectors die here
+            p_src_ = d.s;  // the persistent worlds' records: where the upload put them
+            persist_ = true;
+        }
+    }
     HIP_TRY(sort_temp_reserve(d.sort, std::max<uint32_t>(nc, cell_rows + 1)));
     HIP_TRY(hipMemsetAsync(d.cell_start, 0, sizeof(uint32_t) * ((size_t)cell_rows + 1), d.stream));
     d.allocated = true;
@@ -1598,7 +1665,70 @@ int World::prepare_step() {
 }
 
 // Enqueues n steps (no host synchronisation; safe under stream capture).
+// The persistent batched step: every substep of the n steps for the worlds
+// it runs, in one launch per size class (batch_kernels.hip).  `fresh`: the
+// first of these substeps consumes the one-off user inputs.
+int World::launch_persistent(double delta, uint32_t n, bool fresh, hipStream_t stream) {
+    DeviceWorld& d = d_;
+    const float df = (float)delta;                 // physics.rs:80
+    const float h = df / (float)params_.substeps;  // physics.rs:399
+    PersistArgs k{};
+    k.src = p_src_;
+    k.dst = d.s;
+    k.wstart = d.p_wstart;
+    k.wold_dt = d.wold_dt;
+    k.order = d.p_order;
+    for (uint32_t c = 0; c <= kPersistClasses; ++c) k.class_start[c] = d.p_class_start[c];
+    for (uint32_t c = 0; c < kPersistClasses; ++c) k.groups[c] = d.p_groups[c];
+    k.counter = d.p_counter;
+    k.accum = d.accum;
+    k.block_contacts = d.block_contacts;
+    k.h = h;
+    k.ratio_rest = h / h;
+    k.gx = params_.gravity[0];
+    k.gy = params_.gravity[1];
+    k.nsub = n * params_.substeps;
+    k.fresh = fresh ? 1u : 0u;
+    k.collisions = params_.collisions_enabled ? 1u : 0u;
+    HIP_TRY(launch_batch_persistent(k, stream));
+    p_src_ = d.s;  // the persistent worlds' records are current again
+    return BG_OK;
+}
+
 int World::launch_steps(double delta, uint32_t n) {
+    if (persist_ && path_ == StepPath::Sphere && n > 0) {
+        DeviceWorld& d = d_;
+        const bool fresh = fresh_;
+        if (d.p_grid_worlds == 0) {  // every world on the persistent kernel
+            if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
+            if (int st = launch_persistent(delta, n, fresh, d.stream)) return st;
+            if (int st = stage_end(BG_STAGE_SUBSTEP)) return st;
+            substep_seq_ += n * params_.substeps;
+        } else {
+            // The persistent launch runs beside the grid path's substeps: it touches only its
+            // own worlds' slots and the step's atomic counters, and writes its records into
+            // the buffers the grid path leaves current, which the host knows once it has
+            // queued those substeps.
+            if (!d.p_stream) {
+                HIP_TRY(hipStreamCreateWithFlags(&d.p_stream, hipStreamNonBlocking));
+                HIP_TRY(hipEventCreateWithFlags(&d.p_fork, hipEventDisableTiming));
+                HIP_TRY(hipEventCreateWithFlags(&d.p_join, hipEventDisableTiming));
+            }
+            HIP_TRY(hipEventRecord(d.p_fork, d.stream));
+            HIP_TRY(hipStreamWaitEvent(d.p_stream, d.p_fork, 0));
+            if (int st = launch_grid_steps(delta, n)) return st;
+            if (int st = launch_persistent(delta, n, fresh, d.p_stream)) return st;
+            HIP_TRY(hipEventRecord(d.p_join, d.p_stream));
+            HIP_TRY(hipStreamWaitEvent(d.stream, d.p_join, 0));
+        }
+        fresh_ = false;
+        stepped_since_upload_ = true;
+        return BG_OK;
+    }
+    return launch_grid_steps(delta, n);
+}
+
+int World::launch_grid_steps(double delta, uint32_t n) {
     for (uint32_t k = 0; k < n; ++k) {
         const float df = (float)delta;                             // physics.rs:80
         const float h = df / (float)params_.substeps;              // physics.rs:399

@@ -78,6 +78,9 @@ public:
     int synchronize();
     // n_chunks x chunk_steps steps, each chunk replayed from a captured hipGraph
     int step_chunks(double delta, uint32_t chunk_steps, uint32_t n_chunks);
+    // batches: step with the persistent kernel when the batch allows it (default on)
+    int set_persistent(bool enable);
+    bool persistent_active() const { return persist_; }
     // time candidate broadphase settings over real steps; keep the fastest
     int autotune(double delta, uint32_t steps_per_trial, float* cell_scale, float* skin, uint32_t* interval);
 
@@ -129,6 +132,8 @@ private:
     int run_substep(float h, bool fresh, bool last);
     int prepare_step();
     int launch_steps(double delta, uint32_t n);
+    int launch_grid_steps(double delta, uint32_t n);
+    int launch_persistent(double delta, uint32_t n, bool fresh, hipStream_t stream);
     void advance_time(double delta, uint32_t n);
     bool sphere_eligible() const;
     void dense_slots(std::vector<uint32_t>& bodies, std::vector<uint32_t>& cols) const;
@@ -201,6 +206,11 @@ private:
     uint64_t events_dropped_ = 0;
     // batch: dense body range, old_dt and time of each source world
     std::vector<uint32_t> batch_start_;
+    // batch stepped by the persistent kernel (one workgroup per world): allowed, and in
+    // effect since the last upload (single centred spheres, default groups, no sensors,
+    // constraints or events, worlds of at most kPersistMaxBodies bodies)
+    bool persist_enabled_ = true, persist_ = false;
+    SphereArrays p_src_{};  // where the last persistent launch (or the upload) left its worlds' records
     std::vector<float> batch_old_dt_;
     std::vector<double> batch_time_;
     // profiling: event pairs recorded since the last collection, per stage

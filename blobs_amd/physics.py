@@ -490,6 +490,17 @@ class Physics:
         self._chk(self._lib.bg_batch_size(self._w, C.byref(n)))
         return n.value
 
+    def set_persistent(self, enable: bool) -> None:
+        """Step this batch with the persistent kernel (one workgroup per
+        world, one launch per step call or chunk) when it allows it; the
+        default.  Off: the grid path."""
+        self._chk(self._lib.bg_batch_set_persistent(self._w, int(bool(enable))))
+
+    def persistent_active(self) -> bool:
+        v = C.c_int()
+        self._chk(self._lib.bg_batch_persistent(self._w, C.byref(v)))
+        return bool(v.value)
+
     def writeback(self, worlds=None) -> None:
         """Copy a batch's state back into its source worlds."""
         worlds = list(worlds) if worlds is not None else self._batch_worlds

@@ -222,6 +222,19 @@ int bg_batch_create(bg_world *const *worlds, uint32_t n_worlds, bg_world **out);
 int bg_batch_writeback(bg_world *batch, bg_world *const *worlds, uint32_t n_worlds);
 /* Number of worlds in a batch (0 for a plain world). */
 int bg_batch_size(bg_world *w, uint32_t *out);
+/* Step the batch's worlds of 1,025 to 4,096 bodies with the persistent
+ * kernel (default on): one launch per bg_step / chunk, one workgroup per
+ * world taken from a largest-first queue, every substep of the launch inside
+ * the workgroup with the world's snapshot, grid and state on chip; the other
+ * worlds step on the grid path.  In effect from the next step when the
+ * batch's colliders are centred with default groups and no sensors, and no
+ * collision events are recorded; otherwise (or with enable = 0) every world
+ * steps on the grid path.  Results are the same bits either way.
+ * bg_autotune times both schedules and keeps the faster.  No counterpart in
+ * the reference (it has no batches). */
+int bg_batch_set_persistent(bg_world *w, int enable);
+/* *active = 1 if the batch's last upload gave worlds to the persistent kernel. */
+int bg_batch_persistent(bg_world *w, int *active);
 
 /* ---- state access ---------------------------------------------------- */
 
