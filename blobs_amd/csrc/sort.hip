@@ -33,10 +33,23 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
     return off + incl - v;
 }
 
+// Items to scan: n, or fewer when the device holds the count (the last
+// item's index at *last, e.g. a grid's cell count).
+__device__ __forceinline__ uint32_t scan_len(uint32_t n, const uint32_t* last) {
+    if (!last) return n;
+    const uint32_t m = *last + 1u;
+    return m < n ? m : n;
+}
+
 // sums[tile] = sum of the tile's items (coalesced: item k * 256 + tid).
 __global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
-                                                            uint32_t* __restrict__ sums) {
+                                                            uint32_t* __restrict__ sums, const uint32_t* last) {
     __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    n = scan_len(n, last);
+    if (blockIdx.x * kScanTile >= n) {  // a tile past the items: nothing to read
+        if (threadIdx.x == 0) sums[blockIdx.x] = 0u;
+        return;
+    }
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x;
     uint32_t sum = 0;
 #pragma unroll
@@ -76,10 +89,12 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tile_sums(uint32_t* sums,
 __device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
 
 __global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                             const uint32_t* __restrict__ sums) {
+                                                             const uint32_t* __restrict__ sums, const uint32_t* last) {
     __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
     __shared__ uint32_t s_t[kScanTile + kScanTile / 32];
+    n = scan_len(n, last);
     const uint32_t tile0 = blockIdx.x * kScanTile;
+    if (tile0 >= n) return;  // the whole block leaves together (no barrier is skipped)
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t idx = k * kSortThreads + threadIdx.x;
@@ -241,13 +256,14 @@ void sort_temp_free(SortTemp& t) {
     t = SortTemp{};
 }
 
-hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s) {
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s,
+                              const uint32_t* last) {
     if (n == 0) return hipSuccess;
     uint32_t ntiles = div_up(n, kScanTile);
     if (ntiles > t.max_scan_tiles) return hipErrorInvalidValue;
-    k_tile_sums<<<ntiles, kSortThreads, 0, s>>>(in, n, t.tile_sums);
+    k_tile_sums<<<ntiles, kSortThreads, 0, s>>>(in, n, t.tile_sums, last);
     k_scan_tile_sums<<<1, kSortThreads, 0, s>>>(t.tile_sums, ntiles);
-    k_scan_tiles<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.tile_sums);
+    k_scan_tiles<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.tile_sums, last);
     return hipGetLastError();
 }
 

@@ -25,7 +25,11 @@ hipError_t sort_temp_reserve(SortTemp& t, uint32_t max_items);
 void sort_temp_free(SortTemp& t);
 
 // out[i] = sum(in[0..i)) for i < n.  in and out may alias.
-hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s);
+// out[i] = in[0] + ... + in[i-1] for i < n; with `last`, only the items up to
+// index *last (a device-side count, read by the kernels) are scanned and
+// the outputs past it are left alone.
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s,
+                              const uint32_t* last = nullptr);
 
 // Stable sort by the low `bits` bits of keys.  Sorted data ends in
 // (*keys, *vals); the pointers are swapped with the tmp buffers when the

@@ -72,9 +72,12 @@ __global__ __launch_bounds__(kThreads) void k_cell_scatter(const uint32_t* __res
                                                            uint32_t n_cnt, const float2* __restrict__ abs,
                                                            const float* __restrict__ radius,
                                                            const int32_t* __restrict__ parent,
-                                                           float4* __restrict__ rec) {
+                                                           float4* __restrict__ rec,
+                                                           const GridParams* __restrict__ grid) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t c = i; c < n_cnt; c += gridDim.x * blockDim.x) cnt[c] = 0u;
+    // only the grid's cells were counted (and scanned): clear those
+    const uint32_t n_used = grid->ncells + 1u < n_cnt ? grid->ncells + 1u : n_cnt;
+    for (uint32_t c = i; c < n_used; c += gridDim.x * blockDim.x) cnt[c] = 0u;
     if (i < nc) {
         const uint32_t slot = cell_start[keys[i]] + rank[i];
         if (slot < nc) {  // consistent counts and scan never fail this
@@ -806,12 +809,16 @@ hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity) {
 // Counting sort by cell: cell_start = exclusive scan of the counts (every
 // entry up to cells_cap, so cell_start[ncells] = nc), then the scatter.
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d) {
-    hipError_t e = exclusive_scan_u32(d.cell_count, d.cell_start, d.cells_cap + 1, d.sort, d.stream);
+    // the grid k_rebuild_keys fitted holds ncells <= cells_cap cells: the scan covers those
+    // (the count is read on the device), not the whole table
+    const uint32_t* last =
+        reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(d.grid) + offsetof(GridParams, ncells));
+    hipError_t e = exclusive_scan_u32(d.cell_count, d.cell_start, d.cells_cap + 1, d.sort, d.stream, last);
     if (e != hipSuccess) return e;
     const uint32_t n = d.nc > 0 ? d.nc : 1u;
-    k_cell_scatter<<<div_up(std::max(n, d.cells_cap + 1), kThreads), kThreads, 0, d.stream>>>(
-        d.keys, d.vals_tmp, d.nc, d.cell_start, d.vals, d.cell_count, d.cells_cap + 1, d.c.abs, d.c.radius,
-        d.c.parent, d.crec);
+    k_cell_scatter<<<div_up(n, kThreads), kThreads, 0, d.stream>>>(d.keys, d.vals_tmp, d.nc, d.cell_start, d.vals,
+                                                                   d.cell_count, d.cells_cap + 1, d.c.abs,
+                                                                   d.c.radius, d.c.parent, d.crec, d.grid);
     return hipGetLastError();
 }
 
