@@ -39,7 +39,6 @@ struct Grid {
 struct Shared {
     uint32_t* red;  // [4][waves]
     Grid* grid;
-    uint32_t* item;
 };
 
 // Exclusive scan of cnt[0, n) by the workgroup of NT threads (n <= CELLS + 1).
@@ -432,7 +431,6 @@ __global__ __launch_bounds__(NT) void k_batch_persistent(PersistArgs k, const ui
     Shared sh;
     sh.red = s_red;
     sh.grid = &s_grid;
-    sh.item = &s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();

@@ -680,6 +680,7 @@ int World::ensure_device() {
 
 void World::free_arrays() {
     DeviceWorld& d = d_;
+    if (d.p_stream) (void)hipStreamSynchronize(d.p_stream);
     if (d.stream) (void)hipStreamSynchronize(d.stream);
     {
         BodyArrays* b = &d.b;
