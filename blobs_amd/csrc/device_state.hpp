@@ -164,11 +164,11 @@ struct DeviceWorld {
     // (physics.rs:447-460; the same f32 operations, evaluated at upload)
     float4 *j_mass = nullptr;
     // deferred joint rotations: per solve (component-major, then iteration, then joint),
-    // the axis record and its rotation correction; per body, its joints in arena order
-    // (entry = position in jc_list << 1 | 1 if the body is the joint's b) and component
+    // the axis record (dx, dy, target angle, 1; 0 for a skipped solve); per body, its joints
+    // in arena order (entry = position in jc_list << 1 | 1 if the body is the joint's b) and
+    // component
     uint32_t j_iters = 0;  // joint_iterations the records are sized for
     float4 *jrec = nullptr;
-    float2 *jcorr = nullptr;
     uint32_t *bj_start = nullptr, *bj_list = nullptr;
     int32_t *bj_comp = nullptr;
     uint32_t *jc_start = nullptr, *jc_list = nullptr;

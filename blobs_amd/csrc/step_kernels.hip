@@ -586,8 +586,8 @@ __device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, f
 // every solve's axis goes to jrec[iterations * jc_start[c] + it * nq + q] --
 // record (dx, dy, target angle, 1), or 0 when the anchors coincide and the
 // reference skips the joint -- and the rotations are applied later:
-// k_joint_corr evaluates the atan2 pairs of all records at once and
-// k_integrate adds each body's corrections in sequence order.  Without it
+// k_integrate evaluates each of its body's solves' atan2 pairs and adds the
+// corrections in sequence order.  Without it
 // (joint_iterations changed since the upload) the rotations are applied here.
 __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uint32_t iterations, bool defer) {
     __shared__ float2 s_pos[kJointBodies][64];
@@ -693,16 +693,6 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
     if (flags) atomicOr(&d.accum->error_flags, flags);
 }
 
-// Rotation correction of every joint solve of the substep (physics.rs:462-469),
-// all at once: (corr, 1) for a solve that moved its bodies, (0, 0) for a skipped one.
-__global__ __launch_bounds__(kThreads) void k_joint_corr(const float4* __restrict__ rec, uint32_t n,
-                                                         float2* __restrict__ corr) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const float4 x = rec[r];
-    corr[r] = x.w != 0.0f ? make_float2(joint_rotation_corr(x.x, x.y, x.z), 1.0f) : make_float2(0.0f, 0.0f);
-}
-
 // Verlet + collider transforms + constraints (physics.rs:323-395).
 __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepArgs a) {
     LocalAccum la;
@@ -722,13 +712,14 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
             if (c >= 0) {
                 const uint32_t k0 = d.jc_start[c], nq = d.jc_start[c + 1] - k0;
                 const uint32_t e0 = d.bj_start[b], e1 = d.bj_start[b + 1];
-                const float2* corr = d.jcorr + (size_t)a.joint_iters * k0;
+                const float4* rec = d.jrec + (size_t)a.joint_iters * k0;
                 for (uint32_t it = 0; it < a.joint_iters; ++it)
                     for (uint32_t e = e0; e < e1; ++e) {
                         const uint32_t ent = d.bj_list[e];
-                        const float2 cr = corr[it * nq + ((ent >> 1) - k0)];
-                        if (cr.y == 0.0f) continue;
-                        const float dlt = cr.x * a.h;
+                        const float4 x = rec[it * nq + ((ent >> 1) - k0)];
+                        if (x.w == 0.0f) continue;  // the reference skipped that solve
+                        // both bodies of the solve evaluate its atan2 pair (the same f32 result)
+                        const float dlt = joint_rotation_corr(x.x, x.y, x.z) * a.h;
                         s.rot = (ent & 1u) ? s.rot - dlt : s.rot + dlt;
                         if (!finitef(s.rot)) la.flags |= ERR_JOINT_ROT;
                     }
@@ -863,8 +854,6 @@ hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t
         const bool defer = d.jrec && joint_iterations == d.j_iters;
         k_joints<<<div_up(d.n_jcomp, 64), 64, 0, d.stream>>>(d, a, joint_iterations, defer);
         if (defer) {
-            const uint32_t nrec = joint_iterations * d.nj;
-            k_joint_corr<<<div_up(nrec, kThreads), kThreads, 0, d.stream>>>(d.jrec, nrec, d.jcorr);
             ai.joint_rot = 1u;
             ai.joint_iters = joint_iterations;
         }

@@ -704,7 +704,7 @@ void World::free_arrays() {
     dfree(d.sp_a), dfree(d.sp_b), dfree(d.sp_rest), dfree(d.sp_k), dfree(d.sp_c), dfree(d.bs_start),
         dfree(d.bs_list);
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
-    dfree(d.j_mass), dfree(d.jrec), dfree(d.jcorr), dfree(d.bj_start), dfree(d.bj_list), dfree(d.bj_comp);
+    dfree(d.j_mass), dfree(d.jrec), dfree(d.bj_start), dfree(d.bj_list), dfree(d.bj_comp);
     dfree(d.jc_start), dfree(d.jc_list), dfree(d.jb_start), dfree(d.jb_list), dfree(d.j_local), dfree(d.con_pos),
         dfree(d.con_r);
     dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.crec);
@@ -1256,7 +1256,6 @@ int World::upload() {
     if (n_jcomp > 0 && params_.joint_iterations > 0) {
         d.j_iters = params_.joint_iterations;
         HIP_TRY(dmalloc(&d.jrec, (size_t)d.j_iters * nj));
-        HIP_TRY(dmalloc(&d.jcorr, (size_t)d.j_iters * nj));
         HIP_TRY(dmalloc(&d.bj_start, nb + 1));
         HIP_TRY(dmalloc(&d.bj_list, bj_list.size()));
         HIP_TRY(dmalloc(&d.bj_comp, nb));

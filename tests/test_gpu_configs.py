@@ -3,6 +3,9 @@
 * plain small scene (1,024 spheres + static walls, step(1/60) x 600, contacts
   on): positions, velocities and the contact count bit-exact against the
   oracle's f32 restatement at every 100 steps;
+* plain large scene (1,048,576 spheres, step(1/120) x 100): state digests
+  bit-exact against the f32 oracle every 10 steps and statistics against
+  the FP64 twin (fixture: tests/golden/plain_large_stats.json);
 * multi-collider scene (65,536 bodies, 295,449 colliders, 32,768 springs,
   16,384 fixed joints): bit-exact against the f32 oracle built with the
   device's libm rounding, within 1e-4 r_min of the glibc-built oracle for all
@@ -15,7 +18,7 @@
 * differentiable rollout (256 worlds x 2,048 spheres + 1,024 springs,
   step(1/120) x 200): forward final positions bit-exact against the oracle
   on sampled worlds.
-The plain large scene (1,048,576 spheres) is covered in test_gpu_parity.py.
+The plain large scene's one-step and schedule checks are in test_gpu_parity.py.
 """
 import json
 from pathlib import Path
@@ -146,6 +149,43 @@ def test_multi_collider_300_steps_bitexact_digests_and_fp64_statistics(multi_col
         assert r <= bar, (f"{name}: GPU {gv:.6g} vs FP64 {fv:.6g} ({r:.1%} > {bar:.0%}); "
                           f"the oracle's own f32 run gives {ov:.6g}")
     assert np.isfinite(p.bodies()["position"]).all()
+
+
+def test_plain_large_100_steps_bitexact_digests_and_fp64_statistics():
+    """The plain large scene over its whole horizon (1,048,576 spheres,
+    step(1/120) x 100): every 10 steps the SHA-256 of the GPU state equals
+    the digest of the oracle's f32 restatement (fixture
+    tests/golden/plain_large_stats.json, about 12 minutes of CPU), so the
+    pair sets, corrections and integration are the reference's bit for bit;
+    and the penetration and energy statistics over steps 50-100 are within
+    the bars of the FP64 twin's."""
+    ref = json.loads((GOLDEN / "plain_large_stats.json").read_text())
+    sc = scenes.plain_large_scene(steps=100)
+    assert sc.n_bodies == ref["n_bodies"] == 1 << 20
+    p, _ = sc.build_physics()
+    mass = mass_setup(sc)[0]
+    dyn = sc.body_type == 0
+    gpu = []
+    done = 0
+    for k, s in enumerate(ref["steps"]):
+        p.step(sc.dt, s - done)
+        done = s
+        b = p.bodies()
+        cols = p.collider_positions()
+        assert state_digest(b["position"], b["rotation"], cols) == ref["f32_sha256"][k], f"state after {s} steps"
+        gpu.append(world_stats(cols, sc.col_r, sc.col_parent, b["position"][:, 1], b["velocity"], mass, dyn,
+                               sc.gravity[1]))
+    assert p.step_path() == "sphere"
+    win = [k for k, s in enumerate(ref["steps"]) if s >= 50]
+    g = np.array(gpu)[win].mean(0)
+    f = np.array(ref["fp64"])[win].mean(0)
+    o32 = np.array(ref["f32"])[win].mean(0)
+    assert np.array_equal(np.array(gpu), np.array(ref["f32"])), "statistics of the bit-identical trajectory"
+    rel = np.abs(g - f) / np.abs(f)
+    bars = [0.15, 0.05, 0.08, 0.05, 0.15]
+    for name, r, bar, gv, fv, ov in zip(NAMES, rel, bars, g, f, o32):
+        assert r <= bar, (f"{name}: GPU {gv:.6g} vs FP64 {fv:.6g} ({r:.1%} > {bar:.0%}); "
+                          f"the oracle's own f32 run gives {ov:.6g}")
 
 
 def test_uneven_batch_full_shape_one_chunk_bitexact_on_sampled_worlds():
