@@ -4,8 +4,9 @@
   on): positions, velocities and the contact count bit-exact against the
   oracle's f32 restatement at every 100 steps;
 * plain large scene (1,048,576 spheres, step(1/120) x 100): state digests
-  bit-exact against the f32 oracle every 10 steps and statistics against
-  the FP64 twin (fixture: tests/golden/plain_large_stats.json);
+  bit-exact against the f32 oracle every 10 steps over the whole horizon,
+  statistics against the FP64 twin where the f32 step tracks it (fixture:
+  tests/golden/plain_large_stats.json);
 * multi-collider scene (65,536 bodies, 295,449 colliders, 32,768 springs,
   16,384 fixed joints): bit-exact against the f32 oracle built with the
   device's libm rounding, within 1e-4 r_min of the glibc-built oracle for all
@@ -155,10 +156,16 @@ def test_plain_large_100_steps_bitexact_digests_and_fp64_statistics():
     """The plain large scene over its whole horizon (1,048,576 spheres,
     step(1/120) x 100): every 10 steps the SHA-256 of the GPU state equals
     the digest of the oracle's f32 restatement (fixture
-    tests/golden/plain_large_stats.json, about 12 minutes of CPU), so the
-    pair sets, corrections and integration are the reference's bit for bit;
-    and the penetration and energy statistics over steps 50-100 are within
-    the bars of the FP64 twin's."""
+    tests/golden/plain_large_stats.json, about 14 minutes of CPU), so the
+    pair sets, corrections and integration are the reference's bit for bit,
+    and so are the penetration and energy statistics.
+    Against the FP64 twin: the scene spans coordinates up to 2,140, where an
+    f32 ulp (1.2e-4) exceeds twice gravity's g h^2 = 1.06e-5 and a sizeable
+    part of each displacement, so the f32 step itself -- the reference's as
+    much as this one -- carries, over steps 50-100, 36% less mean
+    penetration, 61% more overlapping pairs and 69% less kinetic energy than
+    FP64 (recorded in the fixture).  The bars apply where the f32 step
+    tracks FP64: maximum penetration (15%) and potential energy (1%)."""
     ref = json.loads((GOLDEN / "plain_large_stats.json").read_text())
     sc = scenes.plain_large_scene(steps=100)
     assert sc.n_bodies == ref["n_bodies"] == 1 << 20
@@ -176,16 +183,13 @@ def test_plain_large_100_steps_bitexact_digests_and_fp64_statistics():
         gpu.append(world_stats(cols, sc.col_r, sc.col_parent, b["position"][:, 1], b["velocity"], mass, dyn,
                                sc.gravity[1]))
     assert p.step_path() == "sphere"
+    assert np.array_equal(np.array(gpu), np.array(ref["f32"])), "statistics of the bit-identical trajectory"
     win = [k for k, s in enumerate(ref["steps"]) if s >= 50]
     g = np.array(gpu)[win].mean(0)
     f = np.array(ref["fp64"])[win].mean(0)
-    o32 = np.array(ref["f32"])[win].mean(0)
-    assert np.array_equal(np.array(gpu), np.array(ref["f32"])), "statistics of the bit-identical trajectory"
     rel = np.abs(g - f) / np.abs(f)
-    bars = [0.15, 0.05, 0.08, 0.05, 0.15]
-    for name, r, bar, gv, fv, ov in zip(NAMES, rel, bars, g, f, o32):
-        assert r <= bar, (f"{name}: GPU {gv:.6g} vs FP64 {fv:.6g} ({r:.1%} > {bar:.0%}); "
-                          f"the oracle's own f32 run gives {ov:.6g}")
+    for q, bar in ((0, 0.15), (4, 0.01)):
+        assert rel[q] <= bar, f"{NAMES[q]}: GPU {g[q]:.6g} vs FP64 {f[q]:.6g} ({rel[q]:.1%} > {bar:.0%})"
 
 
 def test_uneven_batch_full_shape_one_chunk_bitexact_on_sampled_worlds():
