@@ -68,11 +68,12 @@ static __device__ __noinline__ float2 body_xform_slow(float rot, float2 p, float
     return body_xform(rot, p, off);
 }
 
-// Same for a collider whose offset translation is (0,0): R*0 is a signed
-// zero, and x + (+-0) == x for every x != 0, so only zero components need the
-// full evaluation.
+// Same for a collider whose offset translation is (0,0): with a finite
+// rotation R*0 is a signed zero, and x + (+-0) == x for every x != 0, so only
+// zero components need the full evaluation; a rotation that is not finite
+// makes R*0 NaN and the translation NaN, as in the reference.
 __device__ __forceinline__ float2 body_xform_zero_off(float rot, float2 p) {
-    if (p.x != 0.0f && p.y != 0.0f) return p;
+    if (p.x != 0.0f && p.y != 0.0f && finitef(rot)) return p;
     return body_xform_slow(rot, p, make_float2(0.0f, 0.0f));
 }
 

@@ -246,3 +246,28 @@ def test_remove_colliders_mid_run_recomputes_mass():
     o2.step(sc.dt, 3)
     assert p.step_path() == "general"
     assert_bitexact(p.bodies()["position"], o2.pos, "after collider removal")
+
+
+@pytest.mark.parametrize("kind", ["sphere", "general"])
+def test_non_finite_rotation_makes_the_collider_translation_nan(kind):
+    """A body whose rotation is inf or NaN: the reference's collider
+    translation position + R(rotation) * offset is NaN even for a zero offset
+    (physics.rs:360-366), and that collider then touches nothing; every
+    other coordinate stays bit-exact against the oracle."""
+    sc = sphere_scene(seed=26, n=1024) if kind == "sphere" else general_scene(seed=27)
+    p, bh = sc.build_physics()
+    o = OracleWorld(sc, np.float32)
+    for i, v in ((5, np.inf), (9, np.nan), (12, -np.inf)):
+        b = p.get_rbd(int(bh[i]))
+        b.rotation = float(v)
+        p.set_rbd(int(bh[i]), b)
+        o.a["rot"][i] = v
+    p.step(sc.dt, 3)
+    o.step(sc.dt, 3)
+    assert p.step_path() == kind
+    cols = p.collider_positions()
+    nan = np.isnan(o.abs_t)
+    assert nan.any() and np.array_equal(np.isnan(cols), nan)
+    assert_bitexact(np.where(nan, 0.0, cols), np.where(nan, 0.0, o.abs_t), "collider translation")
+    assert_bitexact(p.bodies()["position"], o.pos, "position")
+    assert p.contact_count() == o.contact_count
