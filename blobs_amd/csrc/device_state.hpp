@@ -283,11 +283,28 @@ struct SubstepArgs {
     uint32_t joint_iters;
 };
 
-// Launchers (step_kernels.hip: general path + broadphase build).
-hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity);
+// A general-path rebuild's key pass (step_kernels.hip): the grid from the
+// bounding box accumulated under `parity`, then every collider's cell key
+// and rank.
+struct KeysArgs {
+    const float2* xy;
+    uint32_t nc;
+    StepAccum* accum;
+    uint32_t parity;
+    float cell;
+    uint32_t cells_cap;
+    GridParams* grid_out;
+    uint32_t* keys;
+    uint32_t* rank;
+    uint32_t* cnt;
+    float2* x0;
+};
+
+// Launchers (step_kernels.hip: general path + broadphase build).  A substep
+// that rebuilds passes its key pass (launched with the substep's forces).
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d);
 hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t joint_iterations, bool groups,
-                                  bool sensors);
+                                  bool sensors, const KeysArgs* rebuild);
 // Launchers (sphere_kernels.hip).
 hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity);
 hipError_t launch_sphere_substep(DeviceWorld& d, const SubstepArgs& a, bool groups, bool sensors, bool zero_off,

@@ -1,9 +1,9 @@
 // sort.hip -- exclusive scan and stable LSD radix sort for gfx950.
 //
-// Scan: three launches with no inter-block waiting -- per-tile sums, one
-// block scanning the tile sums, then the per-tile scan offset by its tile's
-// prefix.  The input is read twice; nothing spins on another block's flag,
-// so the cost is bandwidth and launch latency only.  Radix sort: 8-bit digits; per pass a digit
+// Scan: two launches with no inter-block waiting -- per-tile sums, then the
+// per-tile scan, each block first reducing the sums of the tiles before it
+// for its offset.  The input is read twice; nothing spins on another block's
+// flag, so the cost is bandwidth and launch latency only.  Radix sort: 8-bit digits; per pass a digit
 // histogram per 4096-item tile, the scan above over the digit-major matrix,
 // and a scatter that ranks items with wave64 ballots (peer masks from 8
 // ballots, popcount of lower lanes), orders the tile in LDS and writes
@@ -68,21 +68,6 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __re
     }
 }
 
-// In-place exclusive scan of the tile sums by one block.
-__global__ __launch_bounds__(kSortThreads) void k_scan_tile_sums(uint32_t* sums, uint32_t ntiles) {
-    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < ntiles; b += kSortThreads) {
-        const uint32_t i = b + threadIdx.x;
-        const uint32_t v = i < ntiles ? sums[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_excl_scan(v, s_wave, &total);
-        if (i < ntiles) sums[i] = carry + ex;
-        carry += total;
-        __syncthreads();  // s_wave is reused by the next chunk
-    }
-}
-
 // out = exclusive scan of in.  The tile goes through LDS (padded against
 // bank conflicts) so global loads and stores are coalesced while thread t
 // scans the 16 consecutive items [t*16, t*16+16).
@@ -95,6 +80,13 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in,
     n = scan_len(n, last);
     const uint32_t tile0 = blockIdx.x * kScanTile;
     if (tile0 >= n) return;  // the whole block leaves together (no barrier is skipped)
+    // this tile's offset: the sum of the earlier tiles' sums, reduced by the block itself
+    // (no separate pass over the tile sums)
+    uint32_t pre = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSortThreads) pre += sums[t];
+    uint32_t tile_off;
+    (void)block_excl_scan(pre, s_wave, &tile_off);
+    __syncthreads();  // s_wave is reused below
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t idx = k * kSortThreads + threadIdx.x;
@@ -110,7 +102,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in,
         sum += v[k];
     }
     uint32_t total;
-    uint32_t run = sums[blockIdx.x] + block_excl_scan(sum, s_wave, &total);
+    uint32_t run = tile_off + block_excl_scan(sum, s_wave, &total);
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         s_t[scan_pad(base + k)] = run;
@@ -262,7 +254,6 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, Sor
     uint32_t ntiles = div_up(n, kScanTile);
     if (ntiles > t.max_scan_tiles) return hipErrorInvalidValue;
     k_tile_sums<<<ntiles, kSortThreads, 0, s>>>(in, n, t.tile_sums, last);
-    k_scan_tile_sums<<<1, kSortThreads, 0, s>>>(t.tile_sums, ntiles);
     k_scan_tiles<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.tile_sums, last);
     return hipGetLastError();
 }

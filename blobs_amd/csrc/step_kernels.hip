@@ -4,18 +4,23 @@
 // contacts, fixed joints, Verlet update (+ collider transforms), circle
 // constraints.  Two device paths implement it:
 //
-//  * general path (any world): k_forces (gravity + springs, per body),
-//    k_contacts (per body, over its colliders), k_joints (one thread per
-//    independent joint component), k_integrate (Verlet, collider transforms,
-//    constraints, per body);
+//  * general path (any world), per substep: k_keys_forces (the rebuild's
+//    key pass beside gravity + springs per body), the scan and k_cell_scatter
+//    (counting sort of the colliders by cell), k_contact_lists (per collider)
+//    and k_contact_merge (per body; k_contacts for the bodies whose lists
+//    overflowed), the ordered push-out pass (ordered_contacts.hip), k_joints
+//    (one lane per independent joint component), k_integrate (Verlet, joint
+//    rotations, collider transforms, constraints, per body);
 //  * fused sphere path (one collider per body, no springs/joints): all stages
-//    in one kernel per substep over bodies kept in broadphase-cell order.
+//    in one kernel per substep over bodies kept in broadphase-cell order
+//    (sphere_kernels.hip).
 //
-// Contacts are order-independent because the reference tests the collider
-// snapshot written by the previous Verlet (physics.rs:264, 360-366), so the
-// pair set and every correction vector depend only on the snapshot; only the
-// summation order on a body position matters, and each body thread sums its
-// own corrections in the reference's (i, j<i) lexicographic order.
+// Contacts are order-independent until a push-out because the reference
+// tests the collider snapshot written by the previous Verlet (physics.rs:264,
+// 360-366), so the pair set and every correction vector depend only on the
+// snapshot; only the summation order on a body position matters, and each
+// body sums its own corrections in the reference's (i, j<i) lexicographic
+// order.
 #include <float.h>
 
 #include "common.hpp"
@@ -30,34 +35,30 @@ namespace {
 constexpr uint32_t kContactThreads = 64;  // block size of k_contacts (block_contacts: one slot per 64 bodies)
 
 
-// Cell key of every collider snapshot on the grid fitted to the bounding box
+// Cell key of collider i's snapshot on the grid fitted to the bounding box
 // accumulated since the previous rebuild, and its rank among its cell's
 // colliders (counting sort; order inside a cell is arbitrary, no result
-// depends on it); x0 keeps the snapshot.
-__global__ __launch_bounds__(kThreads) void k_rebuild_keys(const float2* __restrict__ xy, uint32_t nc,
-                                                           StepAccum* accum, uint32_t parity, float cell,
-                                                           uint32_t cells_cap, GridParams* grid_out,
-                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
-                                                           uint32_t* cnt, float2* __restrict__ x0) {
-    const GridParams g = grid_from_bbox(accum->bbox[parity], cell, cells_cap);
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// depends on it); x0 keeps the snapshot.  Thread 0 stores the grid and
+// clears the box the coming substep accumulates.
+__device__ __forceinline__ void rebuild_key_at(uint32_t i, const KeysArgs& k) {
+    const GridParams g = grid_from_bbox(k.accum->bbox[k.parity], k.cell, k.cells_cap);
     if (i == 0) {
-        *grid_out = g;
-        accum->bbox[parity ^ 1][0] = 0xFFFFFFFFu;
-        accum->bbox[parity ^ 1][1] = 0xFFFFFFFFu;
-        accum->bbox[parity ^ 1][2] = 0u;
-        accum->bbox[parity ^ 1][3] = 0u;
-        accum->dmax = 0u;
-        accum->n_outliers = 0u;
+        *k.grid_out = g;
+        k.accum->bbox[k.parity ^ 1][0] = 0xFFFFFFFFu;
+        k.accum->bbox[k.parity ^ 1][1] = 0xFFFFFFFFu;
+        k.accum->bbox[k.parity ^ 1][2] = 0u;
+        k.accum->bbox[k.parity ^ 1][3] = 0u;
+        k.accum->dmax = 0u;
+        k.accum->n_outliers = 0u;
     }
-    if (i < nc) {
-        const float2 x = xy[i];
+    if (i < k.nc) {
+        const float2 x = k.xy[i];
         int32_t cx = cell_coord(x.x, g.ox, g.inv_cell, g.nx);
         int32_t cy = cell_coord(x.y, g.oy, g.inv_cell, g.ny);
         const uint32_t kk = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
-        keys[i] = kk;
-        rank[i] = atomicAdd(&cnt[kk], 1u);
-        x0[i] = x;
+        k.keys[i] = kk;
+        k.rank[i] = atomicAdd(&k.cnt[kk], 1u);
+        k.x0[i] = x;
     }
 }
 
@@ -90,9 +91,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_scatter(const uint32_t* __res
 
 // ============================================================ general path
 
-// Gravity + springs -> acc (physics.rs:404-408, springs.rs:25-47).
-__global__ __launch_bounds__(kThreads) void k_forces(DeviceWorld d, SubstepArgs a) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+// Gravity + springs -> acc of body b (physics.rs:404-408, springs.rs:25-47).
+__device__ __forceinline__ void forces_at(uint32_t b, const DeviceWorld& d, const SubstepArgs& a) {
     if (b >= d.nb) return;
     uint32_t fl = d.b.bflags[b];
     if (fl & BF_STATIC) return;
@@ -124,6 +124,19 @@ __global__ __launch_bounds__(kThreads) void k_forces(DeviceWorld d, SubstepArgs 
         }
     }
     d.b.acc[b] = acc;
+}
+
+// The substep's first launch on the general path: the collider keys of the
+// rebuild (blocks [0, key_blocks), when the substep rebuilds) and gravity +
+// springs (the rest).  The two are independent: keys read the collider
+// snapshots, forces the body positions and velocities.
+__global__ __launch_bounds__(kThreads) void k_keys_forces(KeysArgs k, uint32_t key_blocks, DeviceWorld d,
+                                                         SubstepArgs a) {
+    if (blockIdx.x < key_blocks) {
+        rebuild_key_at(blockIdx.x * blockDim.x + threadIdx.x, k);
+    } else if (a.has_springs) {
+        forces_at((blockIdx.x - key_blocks) * blockDim.x + threadIdx.x, d, a);
+    }
 }
 
 constexpr uint32_t kColStage = 8;  // colliders per body staged in LDS by k_contacts
@@ -789,18 +802,10 @@ __global__ __launch_bounds__(kThreads) void k_integrate(DeviceWorld d, SubstepAr
 
 // ============================================================ launchers
 
-hipError_t launch_rebuild_keys(DeviceWorld& d, float cell, uint32_t parity) {
-    uint32_t n = d.nc;
-    uint32_t blocks = div_up(n > 0 ? n : 1, kThreads);
-    k_rebuild_keys<<<blocks, kThreads, 0, d.stream>>>(d.c.abs, n, d.accum, parity, cell, d.cells_cap, d.grid, d.keys,
-                                                      d.vals_tmp, d.cell_count, d.x0);
-    return hipGetLastError();
-}
-
-// Counting sort by cell: cell_start = exclusive scan of the counts (every
-// entry up to cells_cap, so cell_start[ncells] = nc), then the scatter.
+// Counting sort by cell: cell_start = exclusive scan of the counts of the
+// fitted grid's cells (cell_start[ncells] = nc), then the scatter.
 hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d) {
-    // the grid k_rebuild_keys fitted holds ncells <= cells_cap cells: the scan covers those
+    // the grid the key pass fitted holds ncells <= cells_cap cells: the scan covers those
     // (the count is read on the device), not the whole table
     const uint32_t* last =
         reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(d.grid) + offsetof(GridParams, ncells));
@@ -814,10 +819,19 @@ hipError_t launch_rebuild_sort_and_cells(DeviceWorld& d) {
 }
 
 hipError_t launch_general_substep(DeviceWorld& d, const SubstepArgs& a, uint32_t joint_iterations, bool groups,
-                                  bool sensors) {
+                                  bool sensors, const KeysArgs* rebuild) {
     if (d.nb == 0) return hipSuccess;
     const uint32_t blocks = div_up(d.nb, kThreads);
-    if (a.has_springs) k_forces<<<blocks, kThreads, 0, d.stream>>>(d, a);
+    // the rebuild's keys and the substep's forces in one launch, then the rest of the rebuild
+    const uint32_t key_blocks = rebuild ? div_up(rebuild->nc > 0 ? rebuild->nc : 1u, kThreads) : 0u;
+    const uint32_t force_blocks = a.has_springs ? blocks : 0u;
+    if (key_blocks + force_blocks > 0)
+        k_keys_forces<<<key_blocks + force_blocks, kThreads, 0, d.stream>>>(rebuild ? *rebuild : KeysArgs{},
+                                                                            key_blocks, d, a);
+    if (rebuild) {
+        hipError_t e = launch_rebuild_sort_and_cells(d);
+        if (e != hipSuccess) return e;
+    }
     if (a.collisions && d.nc > 0) {
         // One-wave blocks: a body's work varies with its collider count, so small blocks
         // keep the slow waves from holding whole workgroups.  With the collider lists

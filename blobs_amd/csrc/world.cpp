@@ -1418,6 +1418,7 @@ int World::upload() {
     stepped_since_upload_ = false;
     fresh_ = true;
     need_rebuild_ = true;
+    keys_pending_ = false;
     steps_since_rebuild_ = 0;
     return BG_OK;
 }
@@ -1526,8 +1527,10 @@ int World::rebuild() {
     if (sphere) {
         HIP_TRY(launch_sphere_rebuild(d, cell, bbox_parity_));
     } else {
-        HIP_TRY(launch_rebuild_keys(d, cell, bbox_parity_));
-        HIP_TRY(launch_rebuild_sort_and_cells(d));
+        // launched by the substep, beside its forces
+        keys_ = KeysArgs{d.c.abs, d.nc, d.accum, bbox_parity_, cell, d.cells_cap, d.grid, d.keys, d.vals_tmp,
+                         d.cell_count, d.x0};
+        keys_pending_ = true;
     }
     if (int st = stage_end(BG_STAGE_REBUILD)) return st;
     bbox_parity_ ^= 1u;
@@ -1591,7 +1594,9 @@ int World::run_substep(float h, bool fresh, bool last) {
         return BG_OK;
     }
     if (int st = stage_begin(BG_STAGE_SUBSTEP)) return st;
-    HIP_TRY(launch_general_substep(d, a, params_.joint_iterations, groups_, sensors_));
+    const KeysArgs* keys = keys_pending_ ? &keys_ : nullptr;
+    keys_pending_ = false;
+    HIP_TRY(launch_general_substep(d, a, params_.joint_iterations, groups_, sensors_, keys));
     return stage_end(BG_STAGE_SUBSTEP);
 }
 

@@ -170,6 +170,9 @@ private:
     bool stepped_since_upload_ = false;
     bool fresh_ = true;
     bool need_rebuild_ = true;
+    // general path: the rebuild's key pass waits for the substep's first launch
+    bool keys_pending_ = false;
+    KeysArgs keys_{};
     uint32_t steps_since_rebuild_ = 0;
     uint32_t bbox_parity_ = 0;
     float cell_scale_ = 1.0f;
