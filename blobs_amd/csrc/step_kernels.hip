@@ -365,18 +365,23 @@ template <bool GROUPS, bool SENSORS>
 __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d, SubstepArgs a) {
     LocalAccum la;
     la.init();
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0) d.cfb_n[0] = 0u;  // k_contact_merge fills the fallback list after this kernel
-    if (c < d.nc) {
-        const int32_t b = d.c.parent[c];
-        const float2 xa = d.c.abs[c];
+    // threads walk the colliders in cell order (the order the key pass just sorted them
+    // into): a wave's colliders are neighbours, so their candidate records overlap in cache,
+    // and each one's own record is the slot's (x, y, radius, parent), loaded coalesced
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot == 0) d.cfb_n[0] = 0u;  // k_contact_merge fills the fallback list after this kernel
+    if (slot < d.nc) {
+        const uint32_t c = d.vals[slot];
+        const float4 own = d.crec[slot];
+        const int32_t b = __float_as_int(own.w);
+        const float2 xa = make_float2(own.x, own.y);
         uint32_t count = 0, flags = 0;
         ContactList<kColList, uint64_t> L;
         L.init();
         if (b >= 0 && finitef(xa.x) && finitef(xa.y)) {
             const GridParams g = *d.grid;
             const float D = read_dmax(d.accum);
-            const float ra = d.c.radius[c];
+            const float ra = own.z;
             const float mown = d.b.mass[b];
             uint2 ga = make_uint2(0, 0);
             if (GROUPS) ga = d.c.groups[c];
