@@ -152,6 +152,7 @@ struct DeviceWorld {
     uint32_t stat_rep = 1;          // power of two, stat_rep * n_worlds <= 4096
     float2 *wvel = nullptr;         // [n_worlds] mean velocity of the dynamic bodies at the last rebuild
     uint32_t total_cells = 0;       // cell_base[n_worlds]
+    uint32_t *tile_seg = nullptr;   // batch: per scan tile of the cell table, the world holding its first cell
     // springs
     int32_t *sp_a = nullptr, *sp_b = nullptr;
     float *sp_rest = nullptr, *sp_k = nullptr, *sp_c = nullptr;

@@ -3,7 +3,9 @@
 // Scan: two launches with no inter-block waiting -- per-tile sums, then the
 // per-tile scan, each block first reducing the sums of the tiles before it
 // for its offset.  The input is read twice; nothing spins on another block's
-// flag, so the cost is bandwidth and launch latency only.  Radix sort: 8-bit digits; per pass a digit
+// flag, so the cost is bandwidth and launch latency only.  The segmented form
+// (a batch's per-world cell tables) skips tiles no world uses and scans the
+// tile sums in a launch of their own.  Radix sort: 8-bit digits; per pass a digit
 // histogram per 4096-item tile, the scan above over the digit-major matrix,
 // and a scatter that ranks items with wave64 ballots (peer masks from 8
 // ballots, popcount of lower lanes), orders the tile in LDS and writes
@@ -87,6 +89,104 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in,
     uint32_t tile_off;
     (void)block_excl_scan(pre, s_wave, &tile_off);
     __syncthreads();  // s_wave is reused below
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t idx = k * kSortThreads + threadIdx.x;
+        s_t[scan_pad(idx)] = (tile0 + idx < n) ? in[tile0 + idx] : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = threadIdx.x * kSortItems;
+    uint32_t v[kSortItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        v[k] = s_t[scan_pad(base + k)];
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = tile_off + block_excl_scan(sum, s_wave, &total);
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        s_t[scan_pad(base + k)] = run;
+        run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t idx = k * kSortThreads + threadIdx.x;
+        if (tile0 + idx < n) out[tile0 + idx] = s_t[scan_pad(idx)];
+    }
+}
+
+// Whether some segment uses a cell of tile t: only the segment holding the
+// tile's first cell can end before the tile; one starting inside it uses at
+// least its first cell.
+__device__ __forceinline__ bool tile_used(const ScanSegments& g, uint32_t t) {
+    const uint32_t c0 = t * kScanTile, c1 = c0 + kScanTile;
+    const uint32_t w = g.tile_seg[t];
+    if (w >= g.n_seg) return false;
+    if (g.seg_base[w] + g.seg_used[(size_t)w * g.seg_used_stride] + 1u > c0) return true;
+    return w + 1u < g.n_seg && g.seg_base[w + 1u] < c1;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_tile_sums_seg(const uint32_t* __restrict__ in, uint32_t n,
+                                                                uint32_t* __restrict__ sums, ScanSegments g) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    if (!tile_used(g, blockIdx.x)) {
+        if (threadIdx.x == 0) sums[blockIdx.x] = 0u;
+        return;
+    }
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = base + k * kSortThreads;
+        sum += (i < n) ? in[i] : 0u;
+    }
+    sum = wave_sum_u32(sum);
+    if ((threadIdx.x & (BG_WAVE - 1)) == 0) s_wave[threadIdx.x / BG_WAVE] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kSortThreads / BG_WAVE; ++w) t += s_wave[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// sums[0..n) -> their exclusive scan, in place, by one block (kScanTile sums a pass).
+__global__ __launch_bounds__(kSortThreads) void k_scan_sums(uint32_t* sums, uint32_t n) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    uint32_t carry = 0;
+    for (uint32_t p0 = 0; p0 < n; p0 += kScanTile) {
+        const uint32_t base = p0 + threadIdx.x * kSortItems;
+        uint32_t v[kSortItems];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            v[k] = base + k < n ? sums[base + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_excl_scan(sum, s_wave, &total);
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            if (base + k < n) sums[base + k] = run;
+            run += v[k];
+        }
+        carry += total;
+        __syncthreads();  // s_wave is reused by the next pass
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scan_tiles_seg(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                                 const uint32_t* __restrict__ offsets,
+                                                                 ScanSegments g) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    __shared__ uint32_t s_t[kScanTile + kScanTile / 32];
+    if (!tile_used(g, blockIdx.x)) return;  // the whole block leaves together
+    const uint32_t tile0 = blockIdx.x * kScanTile;
+    const uint32_t tile_off = offsets[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t idx = k * kSortThreads + threadIdx.x;
@@ -246,6 +346,17 @@ void sort_temp_free(SortTemp& t) {
     if (t.hist) (void)hipFree(t.hist);
     if (t.tile_sums) (void)hipFree(t.tile_sums);
     t = SortTemp{};
+}
+
+hipError_t exclusive_scan_u32_segments(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s,
+                                       const ScanSegments& g) {
+    if (n == 0) return hipSuccess;
+    const uint32_t ntiles = div_up(n, kScanTile);
+    if (ntiles > t.max_scan_tiles) return hipErrorInvalidValue;
+    k_tile_sums_seg<<<ntiles, kSortThreads, 0, s>>>(in, n, t.tile_sums, g);
+    k_scan_sums<<<1, kSortThreads, 0, s>>>(t.tile_sums, ntiles);
+    k_scan_tiles_seg<<<ntiles, kSortThreads, 0, s>>>(in, out, n, t.tile_sums, g);
+    return hipGetLastError();
 }
 
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s,

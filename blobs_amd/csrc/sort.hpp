@@ -31,6 +31,26 @@ void sort_temp_free(SortTemp& t);
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s,
                               const uint32_t* last = nullptr);
 
+// A cell table cut into segments (one per world of a batch): segment w owns
+// cells [seg_base[w], seg_base[w+1]) and uses the first seg_used[w * stride]
+// + 1 of them this rebuild (its grid's cells and the end sentinel); the rest
+// hold zero counts and nobody reads their starts.  tile_seg[t] is the segment
+// holding the first cell of scan tile t.
+struct ScanSegments {
+    const uint32_t* tile_seg;
+    const uint32_t* seg_base;
+    const uint32_t* seg_used;
+    uint32_t seg_used_stride;
+    uint32_t n_seg;
+};
+
+// exclusive_scan_u32 over such a table: a tile that no segment uses is
+// neither read nor written (its starts keep stale values), and the tile sums
+// are scanned by one block of their own (three launches: the tile count of a
+// large batch makes each tile reducing its predecessors' sums quadratic).
+hipError_t exclusive_scan_u32_segments(const uint32_t* in, uint32_t* out, uint32_t n, SortTemp& t, hipStream_t s,
+                                       const ScanSegments& g);
+
 // Stable sort by the low `bits` bits of keys.  Sorted data ends in
 // (*keys, *vals); the pointers are swapped with the tmp buffers when the
 // number of 8-bit passes is odd.

@@ -294,8 +294,28 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                     }
                 }
             } else if (!GATHER) {
+                // The first eight entries (the first four words, already in registers): their
+                // snapshot loads issue together, without a branch between them.  An entry past
+                // the list loads the body's own snapshot (coalesced) and is masked out.
+                {
+                    uint32_t tt[8];
+                    float4 sb[8];
 #pragma unroll
-                for (uint32_t f = 0; f < kNbrCap; f += 4) {
+                    for (uint32_t u = 0; u < 8; ++u) tt[u] = u < nl ? list_slot(u) : s;
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) sb[u] = k.snap_in[tt[u]];
+#pragma unroll
+                    for (uint32_t u = 0; u < 8; ++u) {
+                        const float dx = xa.x - sb[u].x, dy = xa.y - sb[u].y;
+                        const float rs = ra + sb[u].z;
+                        const bool cand = u < nl && !(__float_as_uint(sb[u].w) & kSnapOutlier) &&
+                                          !(dx * dx + dy * dy > (rs * rs) * 1.00000095367431640625f);
+                        near |= cand ? (1ull << u) : 0ull;
+                    }
+                }
+                // the rest, four a round
+#pragma unroll
+                for (uint32_t f = 8; f < kNbrCap; f += 4) {
                     if (f < nl) {
                         uint32_t tt[4];
                         float4 sb[4];
@@ -860,7 +880,6 @@ struct SphereScatter {
     const uint32_t* __restrict__ rank;
     const uint32_t* __restrict__ cell_start;
     uint32_t* cnt;
-    uint32_t total_cells;
     uint32_t* heads[3];
     uint32_t* bits[3];
     uint32_t* waves[3];
@@ -879,7 +898,6 @@ struct SphereScatter {
 __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = i; c <= k.total_cells; c += stride) k.cnt[c] = 0u;
     for (uint32_t c = i; c < k.n_wstat; c += stride) k.wstat[c] = 0.0f;
     for (uint32_t h = i; h < 3u * k.oh_size; h += stride) {
         const uint32_t t = h / k.oh_size, e = h % k.oh_size;
@@ -888,8 +906,12 @@ __global__ __launch_bounds__(kThreads) void k_sphere_scatter(SphereScatter k) {
     }
     for (uint32_t h = i; h < 3u * k.n_waves; h += stride) k.waves[h / k.n_waves][h % k.n_waves] = 0u;
     if (i >= k.nb) return;
+    // every body counted itself in its key's cell: clearing those cells leaves every count
+    // zero for the next rebuild (the scan has read them)
+    const uint32_t key = k.key[i];
+    k.cnt[key] = 0u;
     if (k.skip && k.skip[k.wid[i]]) return;
-    const uint32_t slot = k.cell_start[k.key[i]] + k.rank[i];
+    const uint32_t slot = k.cell_start[key] + k.rank[i];
     if (slot >= k.nb) return;  // consistent counts and scan never get here
     float4 sn = k.snap[i];
     sn.w = __uint_as_float(__float_as_uint(sn.w) & ~kSnapOutlier);
@@ -932,7 +954,20 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
                                                                  d.cell_count, d.keys, d.vals, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = exclusive_scan_u32(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream);
+    if (d.n_worlds > 1) {
+        // a batch's cell table: only the cells each world's new grid uses
+        ScanSegments g;
+        g.tile_seg = d.tile_seg;
+        g.seg_base = d.cell_base;
+        g.seg_used = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(d.grids) +
+                                                       offsetof(GridParams, ncells));
+        static_assert(sizeof(GridParams) % sizeof(uint32_t) == 0, "GridParams holds 32-bit fields");
+        g.seg_used_stride = sizeof(GridParams) / sizeof(uint32_t);
+        g.n_seg = d.n_worlds;
+        e = exclusive_scan_u32_segments(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream, g);
+    } else {
+        e = exclusive_scan_u32(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream);
+    }
     if (e != hipSuccess) return e;
     SphereScatter k;
     k.snap = d.s.snap;
@@ -943,7 +978,6 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
     k.rank = d.vals;
     k.cell_start = d.cell_start;
     k.cnt = d.cell_count;
-    k.total_cells = d.total_cells;
     for (int t = 0; t < 3; ++t) {
         k.heads[t] = d.oh_head[t];
         k.bits[t] = d.oh_bits[t];

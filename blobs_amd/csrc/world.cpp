@@ -696,7 +696,7 @@ void World::free_arrays() {
     }
     for (int i = 0; i < 3; ++i) dfree(d.oh_head[i]), dfree(d.oh_next[i]), dfree(d.oh_bits[i]), dfree(d.oh_wave[i]);
     dfree(d.cell_count), dfree(d.perm);
-    dfree(d.wid), dfree(d.cell_base), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt), dfree(d.wstat), dfree(d.wvel);
+    dfree(d.wid), dfree(d.cell_base), dfree(d.tile_seg), dfree(d.grids), dfree(d.wbbox), dfree(d.wold_dt), dfree(d.wstat), dfree(d.wvel);
     dfree(d.p_order), dfree(d.p_wstart), dfree(d.p_counter), dfree(d.p_skip);
     for (uint32_t c = 0; c < kPersistClasses; ++c) d.p_groups[c] = 0;
     d.p_grid_worlds = 0;
@@ -1095,7 +1095,10 @@ int World::upload() {
         if (W > 1) wid.resize(nb);
         for (uint32_t w = 0; w < W; ++w) {
             const uint32_t n_w = wstart[w + 1] - wstart[w];
-            const uint64_t cap = std::max<uint64_t>(2ull * n_w, 1024ull);
+            // cells of the world's grid: worlds of a batch spread (nothing bounds them), and a grid
+            // that outgrows its cells doubles its cell size, which crowds the partner walks; a
+            // batch's tables are scanned only where a grid uses them, so room is cheap
+            const uint64_t cap = W > 1 ? std::max<uint64_t>(8ull * n_w, 8192ull) : std::max<uint64_t>(2ull * n_w, 1024ull);
             const uint64_t base = (uint64_t)cell_base[w] + ((cap + 63ull) & ~63ull);
             if (base >= 0xFFFFFFF0ull) return fail(BG_ERR_CAPACITY, "cell table of the batch exceeds 2^32 entries");
             cell_base[w + 1] = (uint32_t)base;
@@ -1123,6 +1126,20 @@ int World::upload() {
         HIP_TRY(hipMemsetAsync(d.wstat, 0, sizeof(float) * 8u * d.stat_rep * W, s));
         HIP_TRY(hipMemsetAsync(d.wvel, 0, sizeof(float2) * W, s));
         HIP_TRY(h2d(d.cell_base, cell_base, s));
+        if (W > 1) {
+            // scan tile -> the world holding its first cell (the last world past the table)
+            const uint32_t ntiles = (uint32_t)(((uint64_t)d.total_cells + 1u + kScanTile - 1u) / kScanTile);
+            std::vector<uint32_t> tile_seg(ntiles);
+            uint32_t w = 0;
+            for (uint32_t t = 0; t < ntiles; ++t) {
+                const uint64_t c0 = (uint64_t)t * kScanTile;
+                while (w + 1u < W && cell_base[w + 1] <= c0) ++w;
+                tile_seg[t] = w;
+            }
+            HIP_TRY(dmalloc(&d.tile_seg, ntiles));
+            HIP_TRY(h2d(d.tile_seg, tile_seg, s));
+            HIP_TRY(hipStreamSynchronize(s));  // the host vector dies here
+        }
         {
             std::vector<uint32_t> wbbox(4u * d.stat_rep * (size_t)W);  // empty boxes, filled by each rebuild
             for (size_t i = 0; i < wbbox.size(); ++i) wbbox[i] = (i & 2u) ? 0u : 0xFFFFFFFFu;
