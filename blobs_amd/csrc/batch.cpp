@@ -299,7 +299,7 @@ int World::autotune(double delta, uint32_t steps_per_trial, float* cell_scale, f
     // cell scale x skin x rebuild interval
     std::vector<Cand> cands;
     for (float cs : {1.0f, 1.25f, 1.5f, 2.0f})
-        for (float sk : {0.25f, 0.5f})
+        for (float sk : {0.25f, 0.35f, 0.5f})
             for (uint32_t iv : {1u, 2u, 3u}) cands.push_back({cs, sk, iv});
     // consume the upload's one-off inputs before timing
     if (fresh_) {

@@ -506,11 +506,22 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
     flush_accum(la, d.accum, d.block_contacts, 0, false);
 }
 
-// Second half: one thread per body gathers its colliders' corrections, sorts
-// them by key -- the reference's (i, j<i) order on this body -- and adds them
-// to the position.  A body with a collider past kColList corrections, or
-// more than kBodyMerge in all, goes to the fallback list (k_contacts in list
-// mode re-walks it).
+// Second half: one thread per body gathers its colliders' corrections and
+// adds them to the position in key order -- the reference's (i, j<i) order on
+// this body.  Each collider's list is already in key order, so a body of at
+// most kMergeRuns colliders stages its lists in LDS (all loads of a round in
+// flight together) and merges them with the runs' heads in registers: one
+// dependent LDS read per correction.  A body of more colliders sorts its
+// staged corrections by insertion.  A body with a collider past kColList
+// corrections, or more than kBodyMerge in all, goes to the fallback list
+// (k_contacts in list mode re-walks it).
+constexpr uint32_t kMergeRuns = 8;   // colliders a body merges as runs
+constexpr uint32_t kMergeBatch = 16;  // corrections staged per round
+
+__device__ __forceinline__ uint64_t merge_key(float4 v) {
+    return ((uint64_t)__float_as_uint(v.x) << 32) | __float_as_uint(v.y);
+}
+
 __global__ __launch_bounds__(kContactThreads) void k_contact_merge(DeviceWorld d) {
     __shared__ float4 s_e[kBodyMerge][kContactThreads];
     const uint32_t lane = threadIdx.x;
@@ -519,6 +530,83 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_merge(DeviceWorld d
     float2 p = d.b.pos[b];
     d.pos_pre[b] = p;  // the ordered pass re-sums from here if a push-out happens
     const uint32_t k0 = d.bc_start[b], k1 = d.bc_start[b + 1];
+    if (k1 - k0 <= kMergeRuns) {
+        const uint32_t nr = k1 - k0;
+        uint32_t cid[kMergeRuns], base[kMergeRuns + 1];
+#pragma unroll
+        for (uint32_t r = 0; r < kMergeRuns; ++r) cid[r] = r < nr ? d.bc_list[k0 + r] : 0u;
+        bool fallback = false;
+        base[0] = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < kMergeRuns; ++r) {
+            const uint32_t n = r < nr ? d.ccnt[cid[r]] : 0u;
+            fallback |= n > kColList;
+            base[r + 1] = base[r] + n;
+        }
+        const uint32_t total = base[kMergeRuns];
+        if (fallback || total > kBodyMerge) {
+            d.cfb_list[atomicAdd(d.cfb_n, 1u)] = b;
+            return;
+        }
+        if (total == 0) return;
+        // stage: entry e of the body is entry e - base[r] of run r (the last run starting at or
+        // before e; an empty run starts where the next one does)
+        for (uint32_t e0 = 0; e0 < total; e0 += kMergeBatch) {
+            float4 v[kMergeBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kMergeBatch; ++u) {
+                const uint32_t e = e0 + u;
+                uint32_t c = cid[0], b0 = 0;
+#pragma unroll
+                for (uint32_t r = 1; r < kMergeRuns; ++r) {
+                    const bool at = base[r] <= e;
+                    c = at ? cid[r] : c;
+                    b0 = at ? base[r] : b0;
+                }
+                v[u] = e < total ? d.clist[(size_t)(e - b0) * d.nc + c] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kMergeBatch; ++u)
+                if (e0 + u < total) s_e[e0 + u][lane] = v[u];
+        }
+        // k-way merge: each run's head key and correction in registers
+        uint64_t hk[kMergeRuns];
+        float2 hv[kMergeRuns];
+        uint32_t pos[kMergeRuns];
+#pragma unroll
+        for (uint32_t r = 0; r < kMergeRuns; ++r) {
+            pos[r] = base[r];
+            const float4 v = base[r] < base[r + 1] ? s_e[base[r]][lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            hk[r] = base[r] < base[r + 1] ? merge_key(v) : ~0ull;
+            hv[r] = make_float2(v.z, v.w);
+        }
+        for (uint32_t m = 0; m < total; ++m) {
+            uint32_t rb = 0;
+            uint64_t kb = hk[0];
+            float2 vb = hv[0];
+#pragma unroll
+            for (uint32_t r = 1; r < kMergeRuns; ++r) {
+                const bool lt = hk[r] < kb;
+                rb = lt ? r : rb;
+                kb = lt ? hk[r] : kb;
+                vb = lt ? hv[r] : vb;
+            }
+            p.x = p.x + vb.x;
+            p.y = p.y + vb.y;
+#pragma unroll
+            for (uint32_t r = 0; r < kMergeRuns; ++r) {
+                if (r == rb) {
+                    const uint32_t q = pos[r] + 1u;
+                    pos[r] = q;
+                    const float4 v = q < base[r + 1] ? s_e[q][lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    hk[r] = q < base[r + 1] ? merge_key(v) : ~0ull;
+                    hv[r] = make_float2(v.z, v.w);
+                }
+            }
+        }
+        d.b.pos[b] = p;
+        return;
+    }
     uint32_t total = 0;
     bool fallback = false;
     for (uint32_t k = k0; k < k1; ++k) {
@@ -540,12 +628,11 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_merge(DeviceWorld d
     // insertion sort by key (runs of one collider are already sorted)
     for (uint32_t i = 1; i < m; ++i) {
         const float4 v = s_e[i][lane];
-        const uint64_t kv = ((uint64_t)__float_as_uint(v.x) << 32) | __float_as_uint(v.y);
+        const uint64_t kv = merge_key(v);
         uint32_t j = i;
         while (j > 0) {
             const float4 u = s_e[j - 1][lane];
-            const uint64_t ku = ((uint64_t)__float_as_uint(u.x) << 32) | __float_as_uint(u.y);
-            if (ku <= kv) break;
+            if (merge_key(u) <= kv) break;
             s_e[j][lane] = u;
             --j;
         }
