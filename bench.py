@@ -91,7 +91,7 @@ def traffic_per_launch(n_bodies: int):
     return rec.get("traffic_bytes_per_launch")
 
 
-def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) -> dict:
+def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool, segments: int = 1) -> dict:
     """The uneven batch: n_worlds single-sphere worlds (N_w log-uniform in
     [64, 16384]), step(1/60) in chunk_steps-step hipGraph chunks; whole-job
     sphere-steps/s over the timed chunks (inputs resident in HBM)."""
@@ -112,14 +112,22 @@ def batched_leg(chunk_steps: int, n_chunks: int, n_worlds: int, autotune: bool) 
         cs, sk, iv = batch.autotune(dt, 4)
         tuned = {"cell_scale": cs, "skin": sk, "rebuild_interval": iv}
         log(f"batched: autotune -> {tuned} ({time.perf_counter() - t0:.1f} s)")
-    def timed(n):
+    def timed(n, segments=1):
         batch.run_chunks(dt, chunk_steps, 1)  # graph capture warm-up
         batch._lib.bg_synchronize(batch._w)
-        t0 = time.perf_counter()
-        batch.run_chunks(dt, chunk_steps, n)  # synchronizes at the end
-        return time.perf_counter() - t0
+        total, left = 0.0, n
+        for k in range(segments):
+            m = left // (segments - k)
+            t0 = time.perf_counter()
+            batch.run_chunks(dt, chunk_steps, m)  # synchronizes at the end
+            el = time.perf_counter() - t0
+            total, left = total + el, left - m
+            if segments > 1:
+                log(f"batched: segment {k}: {m * chunk_steps} steps in {el:.3f} s -> "
+                    f"{n_bodies * m * chunk_steps / el:.4g} sphere-steps/s")
+        return total
 
-    elapsed = timed(n_chunks)
+    elapsed = timed(n_chunks, segments)
     persistent = batch.persistent_active()
     steps = chunk_steps * n_chunks
     if not np.isfinite(batch.bodies()["position"]).all():
@@ -253,6 +261,8 @@ def main() -> int:
     ap.add_argument("--batch-chunks", type=int, default=85,
                     help="timed 10-step chunks (with warm-up and tuning the run stays within 1000 steps)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-segments", type=int, default=1,
+                    help="time the batched chunks in this many consecutive segments, each logged")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -388,7 +398,7 @@ def main() -> int:
         "cpu_baseline": None,
     }
     if not args.no_batched:
-        b = batched_leg(10, args.batch_chunks, args.batch_worlds, not args.no_autotune)
+        b = batched_leg(10, args.batch_chunks, args.batch_worlds, not args.no_autotune, max(1, args.batch_segments))
         b["vs_plain"] = b["value"] / value
         line["batched"] = b
     if not args.no_differentiable:
