@@ -19,6 +19,20 @@ __device__ __forceinline__ float vlen(float x, float y) { return sqrtf(x * x + y
 
 __device__ __forceinline__ bool finitef(float x) { return __builtin_isfinite(x); }
 
+// sqrtf(d2) < r, decided without the square root.  The correctly rounded
+// sqrtf(d2) is below r exactly when sqrt(d2) is below m, the midpoint of r
+// and its predecessor p, i.e. when d2 < m * m: m = (p + r) / 2 and m * m are
+// exact in double (m has at most 26 significant bits), and d2 never equals
+// m * m (m is an odd multiple of half an ulp, so m * m needs more than 24
+// bits).  A NaN or infinite d2 gives false, as sqrtf does; r <= 0 gives false.
+__device__ __forceinline__ bool sqrt_lt(float d2, float r) {
+    if (!(r > 0.0f)) return false;
+    if (!finitef(r)) return d2 < r;
+    const float p = __uint_as_float(__float_as_uint(r) - 1u);
+    const double m = ((double)p + (double)r) * 0.5;
+    return (double)d2 < m * m;
+}
+
 // Order-preserving map float -> uint32 for atomicMin/atomicMax.
 __device__ __forceinline__ uint32_t f2ord(float f) {
     uint32_t u = __float_as_uint(f);

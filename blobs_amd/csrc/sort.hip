@@ -10,6 +10,8 @@
 // and a scatter that ranks items with wave64 ballots (peer masks from 8
 // ballots, popcount of lower lanes), orders the tile in LDS and writes
 // digit runs out contiguously.
+#include <stdlib.h>
+
 #include "common.hpp"
 #include "sort.hpp"
 
@@ -427,6 +429,70 @@ extern "C" int bgx_exclusive_scan_u32(const uint32_t* in, uint32_t n, uint32_t* 
 done:
     (void)hipFree(d);
     sort_temp_free(t);
+    return (int)e;
+}
+
+// The segmented scan over a host table: segment w owns [seg_base[w], seg_base[w+1])
+// and uses its first seg_used[w] + 1 cells; outputs elsewhere are left as `out` held them.
+extern "C" int bgx_exclusive_scan_u32_segments(const uint32_t* in, uint32_t n, const uint32_t* seg_base,
+                                               const uint32_t* seg_used, uint32_t n_seg, uint32_t* out) {
+    if (n_seg == 0 || seg_base[n_seg] + 1u != n) return (int)hipErrorInvalidValue;
+    const uint32_t ntiles = div_up(n, kScanTile);
+    uint32_t *d_in = nullptr, *d_out = nullptr, *d_base = nullptr, *d_used = nullptr, *d_tile = nullptr;
+    SortTemp t;
+    hipError_t e = hipSuccess;
+    uint32_t* tile = (uint32_t*)malloc(sizeof(uint32_t) * ntiles);
+    ScanSegments g;
+    if (!tile) return (int)hipErrorOutOfMemory;
+    for (uint32_t k = 0, w = 0; k < ntiles; ++k) {
+        while (w + 1u < n_seg && seg_base[w + 1] <= k * (uint32_t)kScanTile) ++w;
+        tile[k] = w;
+    }
+    if ((e = hipMalloc(&d_in, sizeof(uint32_t) * n)) || (e = hipMalloc(&d_out, sizeof(uint32_t) * n)) ||
+        (e = hipMalloc(&d_base, sizeof(uint32_t) * (n_seg + 1))) || (e = hipMalloc(&d_used, sizeof(uint32_t) * n_seg)) ||
+        (e = hipMalloc(&d_tile, sizeof(uint32_t) * ntiles)) || (e = sort_temp_reserve(t, n)))
+        goto done;
+    if ((e = hipMemcpy(d_in, in, sizeof(uint32_t) * n, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(d_out, out, sizeof(uint32_t) * n, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(d_base, seg_base, sizeof(uint32_t) * (n_seg + 1), hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(d_used, seg_used, sizeof(uint32_t) * n_seg, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(d_tile, tile, sizeof(uint32_t) * ntiles, hipMemcpyHostToDevice)))
+        goto done;
+    g.tile_seg = d_tile;
+    g.seg_base = d_base;
+    g.seg_used = d_used;
+    g.seg_used_stride = 1;
+    g.n_seg = n_seg;
+    if ((e = exclusive_scan_u32_segments(d_in, d_out, n, t, nullptr, g))) goto done;
+    if ((e = hipDeviceSynchronize())) goto done;
+    e = hipMemcpy(out, d_out, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+done:
+    free(tile);
+    (void)hipFree(d_in), (void)hipFree(d_out), (void)hipFree(d_base), (void)hipFree(d_used), (void)hipFree(d_tile);
+    sort_temp_free(t);
+    return (int)e;
+}
+
+__global__ __launch_bounds__(256) void k_sqrt_lt_check(const float* d2, const float* r, uint32_t n, uint8_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (uint8_t)((sqrtf(d2[i]) < r[i] ? 1u : 0u) | (sqrt_lt(d2[i], r[i]) ? 2u : 0u));
+}
+
+// For each pair: bit 0 = (sqrtf(d2) < r), bit 1 = sqrt_lt(d2, r), both evaluated on the device.
+extern "C" int bgx_sqrt_lt(const float* d2, const float* r, uint32_t n, uint8_t* out) {
+    float *a = nullptr, *b = nullptr;
+    uint8_t* o = nullptr;
+    hipError_t e = hipSuccess;
+    const size_t m = n ? n : 1;
+    if ((e = hipMalloc(&a, 4 * m)) || (e = hipMalloc(&b, 4 * m)) || (e = hipMalloc(&o, m))) goto done;
+    if ((e = hipMemcpy(a, d2, 4ull * n, hipMemcpyHostToDevice)) || (e = hipMemcpy(b, r, 4ull * n, hipMemcpyHostToDevice)))
+        goto done;
+    if (n) k_sqrt_lt_check<<<div_up(n, 256u), 256>>>(a, b, n, o);
+    if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) goto done;
+    e = hipMemcpy(out, o, n, hipMemcpyDeviceToHost);
+done:
+    (void)hipFree(a), (void)hipFree(b), (void)hipFree(o);
     return (int)e;
 }
 

@@ -207,17 +207,16 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
                     // the sign of a zero, so the unoriented difference decides and the axis
                     // is formed for a hit only
                     bool hit[4];
-                    float dx[4], dy[4], dist[4], rs[4];
+                    float dx[4], dy[4], dot[4], rs[4];
 #pragma unroll
                     for (uint32_t u = 0; u < 4; ++u) {
                         const int32_t pj = __float_as_int(rb[u].w);
                         dx[u] = xa.x - rb[u].x;
                         dy[u] = xa.y - rb[u].y;
                         rs[u] = ra + rb[u].z;
-                        const float dot = dx[u] * dx[u] + dy[u] * dy[u];
-                        dist[u] = sqrtf(dot);
+                        dot[u] = dx[u] * dx[u] + dy[u] * dy[u];
                         hit[u] = u < nv && pj >= 0 && pj != (int32_t)b &&
-                                 !(dot > (rs[u] * rs[u]) * 1.00000095367431640625f) && dist[u] < rs[u];
+                                 !(dot[u] > (rs[u] * rs[u]) * 1.00000095367431640625f) && sqrt_lt(dot[u], rs[u]);
                     }
                     uint32_t jj[4];
                     float mp[4];
@@ -241,7 +240,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(DeviceWorld d, Sub
                         const bool isA = c > j;
                         const float ax = isA ? dx[u] : rb[u].x - xa.x, ay = isA ? dy[u] : rb[u].y - xa.y;
                         const uint64_t key = isA ? (((uint64_t)c << 32) | j) : (((uint64_t)j << 32) | c);
-                        emit(key, isA, j, __float_as_int(rb[u].w), ax, ay, dist[u], rs[u], sen_a, mp[u]);
+                        emit(key, isA, j, __float_as_int(rb[u].w), ax, ay, sqrtf(dot[u]), rs[u], sen_a, mp[u]);
                     }
                 });
             }
@@ -376,7 +375,9 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
         const int32_t b = __float_as_int(own.w);
         const float2 xa = make_float2(own.x, own.y);
         uint32_t count = 0, flags = 0;
-        ContactList<kColList, uint64_t> L;
+        // keyed by the partner collider j: for this collider c, the pair key (max, min) of
+        // (c, j) orders as j does (every j < c before every j > c, each side by j)
+        ContactList<kColList, uint32_t> L;
         L.init();
         if (b >= 0 && finitef(xa.x) && finitef(xa.y)) {
             const GridParams g = *d.grid;
@@ -391,17 +392,16 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
                 scan_stencil(g, d.cell_start, d.nc, d.crec, xa.x - a.drift_prev_x, xa.y - a.drift_prev_y, Q,
                              [&](uint32_t t, const float4 (&rb)[4], uint32_t nv) {
                     bool hit[4];
-                    float dx[4], dy[4], dist[4], rs[4];
+                    float dx[4], dy[4], dot[4], rs[4];
 #pragma unroll
                     for (uint32_t u = 0; u < 4; ++u) {
                         const int32_t pj = __float_as_int(rb[u].w);
                         dx[u] = xa.x - rb[u].x;
                         dy[u] = xa.y - rb[u].y;
                         rs[u] = ra + rb[u].z;
-                        const float dot = dx[u] * dx[u] + dy[u] * dy[u];
-                        dist[u] = sqrtf(dot);
+                        dot[u] = dx[u] * dx[u] + dy[u] * dy[u];
                         hit[u] = u < nv && pj >= 0 && pj != b &&
-                                 !(dot > (rs[u] * rs[u]) * 1.00000095367431640625f) && dist[u] < rs[u];
+                                 !(dot[u] > (rs[u] * rs[u]) * 1.00000095367431640625f) && sqrt_lt(dot[u], rs[u]);
                     }
                     uint32_t jj[4];
                     float mp[4];
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
                         const bool isA = c > j;
                         const float ax = isA ? dx[u] : rb[u].x - xa.x, ay = isA ? dy[u] : rb[u].y - xa.y;
                         const uint64_t key = isA ? (((uint64_t)c << 32) | j) : (((uint64_t)j << 32) | c);
-                        emit(key, isA, j, __float_as_int(rb[u].w), ax, ay, dist[u], rs[u], mp[u]);
+                        emit(key, isA, j, __float_as_int(rb[u].w), ax, ay, sqrtf(dot[u]), rs[u], mp[u]);
                     }
                 });
             };
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
                 }
                 const bool sen_b = SENSORS ? (d.c.cflags[j] & 1u) : false;
                 if (sen_a || sen_b) {
-                    if (pushed) L.add(key, px, py);
+                    if (pushed) L.add(j, px, py);
                     return;
                 }
                 const float mA = isA ? mown : mpart, mB = isA ? mpart : mown;
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
                     cx = px + cx;
                     cy = py + cy;
                 }
-                L.add(key, cx, cy);
+                L.add(j, cx, cy);
             });
             if (d.events && count > 0) {
                 // collision events of the pairs whose A collider is this one (physics.rs:288-311)
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kContactThreads) void k_contact_lists(DeviceWorld d
         for (uint32_t i = 0; i < kColList; ++i)
             if (i < nk)
                 d.clist[(size_t)i * d.nc + c] =
-                    make_float4(__uint_as_float((uint32_t)(L.key[i] >> 32)), __uint_as_float((uint32_t)L.key[i]),
+                    make_float4(__uint_as_float(c > L.key[i] ? c : L.key[i]), __uint_as_float(c > L.key[i] ? L.key[i] : c),
                                 L.cx[i], L.cy[i]);
         d.ccnt[c] = L.n;
         la.contacts = count;
