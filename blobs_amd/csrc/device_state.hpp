@@ -179,6 +179,10 @@ struct DeviceWorld {
     // jc_list, the joint's two bodies as local indices (a | b << 8)
     uint32_t *jb_start = nullptr, *j_local = nullptr;
     int32_t *jb_list = nullptr;
+    // per entry of jc_list: the joint's anchors (a.x, a.y, b.x, b.y), mass terms and
+    // (distance, target angle), so that a block stages its components' joints coalesced
+    float4 *jq_anchor = nullptr, *jq_mass = nullptr;
+    float2 *jq_len = nullptr;
     // constraints
     float2 *con_pos = nullptr;
     float *con_r = nullptr;

@@ -681,11 +681,15 @@ __device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, f
 
 // Fixed joints (physics.rs:424-477), positions: Gauss-Seidel in arena order
 // inside each connected component of the joint graph (components touch
-// disjoint bodies, so they run in parallel, one lane each).  A component of
-// at most kJointBodies bodies and kJointJoints joints iterates on copies of
-// its bodies and joints in LDS (the sweep is a chain of dependent reads and
-// writes, so each step costs an LDS round trip instead of a cache round trip)
-// and writes its bodies back once; a larger one iterates in global memory.
+// disjoint bodies, so they run in parallel, one lane each).  A block of 64
+// components first stages, with coalesced loads, the constant records of all
+// their joints (stored in component order at upload) and the positions of
+// the bodies of its components of at most kJointBodies bodies and
+// kJointJoints joints; each such component then iterates on those LDS copies
+// (the sweep is a chain of dependent reads and writes, so each step costs an
+// LDS round trip instead of a cache round trip), and the block writes its
+// bodies back.  A larger component, or every component of a block whose
+// joints or bodies exceed the block's LDS, iterates in global memory.
 // Writes keep the reference's order (a's position, b's, a's rotation, b's),
 // which decides a joint of a body to itself.  With `defer`,
 // every solve's axis goes to jrec[iterations * jc_start[c] + it * nq + q] --
@@ -694,82 +698,72 @@ __device__ __forceinline__ bool joint_solve(float2& pa, float2& pb, float& ra, f
 // k_integrate evaluates each of its body's solves' atan2 pairs and adds the
 // corrections in sequence order.  Without it
 // (joint_iterations changed since the upload) the rotations are applied here.
+constexpr uint32_t kBlockJoints = 512;  // joints of a block's components staged in LDS
+constexpr uint32_t kBlockBodies = 640;  // bodies of its small components staged in LDS
+
 __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uint32_t iterations, bool defer) {
-    __shared__ float2 s_pos[kJointBodies][64];
-    __shared__ float s_rot[kJointBodies][64];
-    __shared__ float4 s_jm[kJointJoints][64];      // mass terms of the joint
-    __shared__ int32_t s_gid[kJointBodies][64];    // body index
-    __shared__ float4 s_anchor[kJointJoints][64];  // anchor a, anchor b
-    __shared__ float2 s_len[kJointJoints][64];     // distance, target angle
-    __shared__ uint32_t s_loc[kJointJoints][64];   // local a | local b << 8
+    __shared__ float4 s_anchor[kBlockJoints];  // anchor a, anchor b
+    __shared__ float4 s_jm[kBlockJoints];      // mass terms
+    __shared__ float2 s_len[kBlockJoints];     // distance, target angle
+    __shared__ uint32_t s_loc[kBlockJoints];   // local a | local b << 8
+    __shared__ float2 s_pos[kBlockBodies];
+    __shared__ float s_rot[kBlockBodies];
+    __shared__ int32_t s_gid[kBlockBodies];    // body index
+    __shared__ uint8_t s_stat[kBlockBodies];   // 1: static body
     const uint32_t lane = threadIdx.x;
-    const uint32_t comp = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c0 = blockIdx.x * 64u, c1 = min(c0 + 64u, d.n_jcomp);
+    const uint32_t comp = c0 + lane;
+    const uint32_t q0 = d.jc_start[c0], q1 = d.jc_start[c1];
+    const uint32_t b0 = d.jb_start[c0], b1 = d.jb_start[c1];
+    const bool staged = q1 - q0 <= kBlockJoints && b1 - b0 <= kBlockBodies;  // the same for the block
     uint32_t flags = 0;
-    if (comp < d.n_jcomp) {
+    if (staged) {
+        for (uint32_t i = lane; i < q1 - q0; i += 64u) {
+            s_anchor[i] = d.jq_anchor[q0 + i];
+            s_jm[i] = d.jq_mass[q0 + i];
+            s_len[i] = d.jq_len[q0 + i];
+            s_loc[i] = d.j_local[q0 + i];
+        }
+        for (uint32_t i = lane; i < b1 - b0; i += 64u) {
+            const int32_t g = d.jb_list[b0 + i];
+            s_gid[i] = g;
+            s_pos[i] = d.b.pos[g];
+            if (!defer) s_rot[i] = d.b.rot[g];
+            s_stat[i] = (d.b.bflags[g] & BF_STATIC) ? 1u : 0u;
+        }
+        __syncthreads();
+    }
+    if (comp < c1) {
         const uint32_t k0 = d.jc_start[comp], k1 = d.jc_start[comp + 1];
-        const uint32_t b0 = d.jb_start[comp], nbl = d.jb_start[comp + 1] - b0;
+        const uint32_t lb = d.jb_start[comp] - b0, nbl = d.jb_start[comp + 1] - d.jb_start[comp];
         const uint32_t nq = k1 - k0;
         float4* rec = defer ? d.jrec + (size_t)iterations * k0 : nullptr;
-        if (nbl > 0) {
-            // copies in, eight entries per round so that their loads overlap (an index past
-            // the end repeats the last entry, rewriting its slot with the same values)
-            uint32_t stat = 0;  // bit l: local body l is static
-            for (uint32_t l0 = 0; l0 < nbl; l0 += 8) {
-                int32_t g[8];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; ++u) g[u] = d.jb_list[b0 + min(l0 + u, nbl - 1)];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; ++u) {
-                    const uint32_t l = min(l0 + u, nbl - 1);
-                    s_gid[l][lane] = g[u];
-                    s_pos[l][lane] = d.b.pos[g[u]];
-                    if (!defer) s_rot[l][lane] = d.b.rot[g[u]];
-                    if (d.b.bflags[g[u]] & BF_STATIC) stat |= 1u << l;
-                }
-            }
-            for (uint32_t q0 = 0; q0 < nq; q0 += 8) {
-                uint32_t j[8];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; ++u) j[u] = d.jc_list[k0 + min(q0 + u, nq - 1)];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; ++u) {
-                    const uint32_t q = min(q0 + u, nq - 1);
-                    const float2 an_a = d.j_anchor_a[j[u]], an_b = d.j_anchor_b[j[u]];
-                    s_anchor[q][lane] = make_float4(an_a.x, an_a.y, an_b.x, an_b.y);
-                    s_len[q][lane] = make_float2(d.j_dist[j[u]], d.j_target[j[u]]);
-                    s_jm[q][lane] = d.j_mass[j[u]];
-                    s_loc[q][lane] = d.j_local[k0 + q];
-                }
-            }
+        if (staged && nbl > 0) {
+            const uint32_t lq = k0 - q0;
             for (uint32_t it = 0; it < iterations; ++it) {
                 for (uint32_t q = 0; q < nq; ++q) {
-                    const uint32_t loc = s_loc[q][lane];
-                    const uint32_t la = loc & 0xFFu, lb = (loc >> 8) & 0xFFu;
-                    const float4 an = s_anchor[q][lane];
-                    const float2 len = s_len[q][lane];
-                    float2 pa = s_pos[la][lane], pb = s_pos[lb][lane];
+                    const uint32_t loc = s_loc[lq + q];
+                    const uint32_t la = lb + (loc & 0xFFu), lbb = lb + ((loc >> 8) & 0xFFu);
+                    const float4 an = s_anchor[lq + q];
+                    const float2 len = s_len[lq + q];
+                    float2 pa = s_pos[la], pb = s_pos[lbb];
                     float dx, dy;
                     const bool moved = joint_positions(pa, pb, make_float2(an.x, an.y), make_float2(an.z, an.w),
-                                                       len.x, s_jm[q][lane], (stat >> la) & 1u, (stat >> lb) & 1u,
+                                                       len.x, s_jm[lq + q], s_stat[la] != 0u, s_stat[lbb] != 0u,
                                                        dx, dy, flags);
                     if (defer) rec[it * nq + q] = moved ? make_float4(dx, dy, len.y, 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
                     if (!moved) continue;
-                    s_pos[la][lane] = pa;
-                    s_pos[lb][lane] = pb;
+                    s_pos[la] = pa;
+                    s_pos[lbb] = pb;
                     if (!defer) {
-                        float ra = s_rot[la][lane], rb = s_rot[lb][lane];
+                        float ra = s_rot[la], rb = s_rot[lbb];
                         joint_rotate(ra, rb, joint_rotation_corr(dx, dy, len.y), a.h, flags);
-                        s_rot[la][lane] = ra;
-                        s_rot[lb][lane] = rb;
+                        s_rot[la] = ra;
+                        s_rot[lbb] = rb;
                     }
                 }
             }
-            for (uint32_t l = 0; l < nbl; ++l) {
-                const int32_t g = s_gid[l][lane];
-                d.b.pos[g] = s_pos[l][lane];
-                if (!defer) d.b.rot[g] = s_rot[l][lane];
-            }
-        } else {
+        } else if (!staged || nbl == 0) {
             for (uint32_t it = 0; it < iterations; ++it) {
                 for (uint32_t kk = k0; kk < k1; ++kk) {
                     const uint32_t j = d.jc_list[kk];
@@ -793,6 +787,14 @@ __global__ __launch_bounds__(64) void k_joints(DeviceWorld d, SubstepArgs a, uin
                     }
                 }
             }
+        }
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t i = lane; i < b1 - b0; i += 64u) {
+            const int32_t g = s_gid[i];
+            d.b.pos[g] = s_pos[i];
+            if (!defer) d.b.rot[g] = s_rot[i];
         }
     }
     if (flags) atomicOr(&d.accum->error_flags, flags);

@@ -705,7 +705,7 @@ void World::free_arrays() {
         dfree(d.bs_list);
     dfree(d.j_a), dfree(d.j_b), dfree(d.j_anchor_a), dfree(d.j_anchor_b), dfree(d.j_dist), dfree(d.j_target);
     dfree(d.j_mass), dfree(d.jrec), dfree(d.bj_start), dfree(d.bj_list), dfree(d.bj_comp);
-    dfree(d.jc_start), dfree(d.jc_list), dfree(d.jb_start), dfree(d.jb_list), dfree(d.j_local), dfree(d.con_pos),
+    dfree(d.jc_start), dfree(d.jc_list), dfree(d.jb_start), dfree(d.jb_list), dfree(d.j_local), dfree(d.jq_anchor), dfree(d.jq_mass), dfree(d.jq_len), dfree(d.con_pos),
         dfree(d.con_r);
     dfree(d.keys), dfree(d.vals), dfree(d.keys_tmp), dfree(d.vals_tmp), dfree(d.cell_start), dfree(d.crec);
     dfree(d.grid), dfree(d.accum), dfree(d.block_contacts), dfree(d.events), dfree(d.vel_next);
@@ -1002,6 +1002,16 @@ int World::upload() {
             jb_start[c + 1] = (uint32_t)jb_list.size();
         }
     }
+    // the joints' constant records in component order (entry kk of jc_list), for the joint
+    // stage's coalesced staging
+    std::vector<float4> jq_anchor(jc_list.size()), jq_mass(jc_list.size());
+    std::vector<float2> jq_len(jc_list.size());
+    for (size_t kk = 0; kk < jc_list.size(); ++kk) {
+        const uint32_t j = jc_list[kk];
+        jq_anchor[kk] = make_float4(j_an_a[j].x, j_an_a[j].y, j_an_b[j].x, j_an_b[j].y);
+        jq_mass[kk] = j_mass[j];
+        jq_len[kk] = make_float2(j_dist[j], j_target[j]);
+    }
     // per body: its joints in arena order and its component (deferred joint rotations)
     std::vector<uint32_t> bj_start(nb + 1, 0), bj_list;
     std::vector<int32_t> bj_comp(nb, -1);
@@ -1282,6 +1292,9 @@ int World::upload() {
     HIP_TRY(dmalloc(&d.jb_start, n_jcomp + 1));
     HIP_TRY(dmalloc(&d.jb_list, jb_list.size()));
     HIP_TRY(dmalloc(&d.j_local, nj));
+    HIP_TRY(dmalloc(&d.jq_anchor, jq_anchor.size()));
+    HIP_TRY(dmalloc(&d.jq_mass, jq_mass.size()));
+    HIP_TRY(dmalloc(&d.jq_len, jq_len.size()));
     HIP_TRY(dmalloc(&d.con_pos, nk));
     HIP_TRY(dmalloc(&d.con_r, nk));
     HIP_TRY(dmalloc(&d.keys, nc));
@@ -1421,6 +1434,9 @@ int World::upload() {
     HIP_TRY(h2d(d.jb_start, jb_start, s));
     HIP_TRY(h2d(d.jb_list, jb_list, s));
     HIP_TRY(h2d(d.j_local, j_local, s));
+    HIP_TRY(h2d(d.jq_anchor, jq_anchor, s));
+    HIP_TRY(h2d(d.jq_mass, jq_mass, s));
+    HIP_TRY(h2d(d.jq_len, jq_len, s));
     HIP_TRY(h2d(d.con_pos, con_pos, s));
     HIP_TRY(h2d(d.con_r, con_r, s));
     StepAccum acc0;
