@@ -1788,7 +1788,8 @@ int World::launch_grid_steps(double delta, uint32_t n) {
             // rebuild (one fast collider would slow every query): it re-sorts every substep
             if (path_ == StepPath::General) need_rebuild_ = true;
         }
-        if (++steps_since_rebuild_ >= rebuild_interval_) need_rebuild_ = true;
+        const uint32_t interval = rebuild_interval_ ? rebuild_interval_ : (d_.nb < 65536u ? 1u : 3u);
+        if (++steps_since_rebuild_ >= interval) need_rebuild_ = true;
     }
     return BG_OK;
 }

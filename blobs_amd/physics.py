@@ -603,7 +603,7 @@ class Physics:
         self._chk(self._lib.bg_get_collider_positions(self._w, L.fptr(out), C.byref(n)))
         return out
 
-    def set_broadphase(self, cell_scale: float = 1.0, rebuild_interval: int = 1, skin: float = 0.5) -> None:
+    def set_broadphase(self, cell_scale: float = 1.25, rebuild_interval: int = 1, skin: float = 0.25) -> None:
         """Broadphase tuning (results never depend on it): cell size as a
         multiple of 2*max radius, steps between grid rebuilds, and the skin
         (fraction of a cell) beyond which a body is tracked as an outlier."""

@@ -301,7 +301,9 @@ int bg_debug_data(bg_world *w, bg_affine2 *bodies, bg_affine2 *colliders, float 
  * (>= 0.25), the skin (fraction of a cell a body may drift from its
  * rebuild position before it is tracked as an outlier; fused sphere path)
  * and the number of steps between grid rebuilds (>= 1).  Results do not
- * depend on any of them; only speed does. */
+ * depend on any of them; only speed does.  Until it is called: cell scale
+ * 1.25, skin 0.25, a rebuild every step below 65,536 bodies and every third
+ * step from there. */
 int bg_set_broadphase(bg_world *w, float cell_scale, float skin, uint32_t rebuild_interval);
 /* Which device path the last step ran: 0 none, 1 general, 2 fused spheres. */
 int bg_step_path(bg_world *w, uint32_t *out);
