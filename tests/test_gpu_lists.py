@@ -73,3 +73,34 @@ def test_long_grid_rows_overflow_slot_deltas():
     assert p.contact_count() == o.contact_count > 10000
     h = list_lengths(p)
     assert h[-1] > 1000 and h[1:-1].sum() > 1000, f"both lists and far-partner fallbacks expected: {h}"
+
+
+def test_spreading_batch_keeps_its_lists():
+    """Worlds without walls spread as they fall.  Here each world's dense core
+    (packing 0.5) spreads slowly while 5% of its bodies fly off at 100
+    units/s, stretching its bounding box: with 2 cells per body the grids
+    double their cell size and, after 150 steps, about 9% of the bodies have
+    lost their partner lists (flat walks past 128 candidates); a batch
+    world's 8 cells per body keep them (about 0.2% without).  The results
+    equal each world stepped alone (2 cells per body, other grids and lists)."""
+    from blobs_amd.physics import Physics
+    rng = scenes.Rng(90)
+    scs = [scenes.uneven_batch_world(2000, 0.5, rng, vel_sigma=0.2) for _ in range(12)]
+    for w, sc in enumerate(scs):
+        pick = np.random.default_rng(w).choice(sc.n_bodies, sc.n_bodies // 20, replace=False)
+        ang = np.random.default_rng(100 + w).uniform(0.0, 2.0 * np.pi, pick.size)
+        sc.vreq[pick] = (100.0 * np.stack([np.cos(ang), np.sin(ang)], 1)).astype(np.float32)
+    b = Physics.batch([sc.build_physics()[0] for sc in scs])
+    b.set_persistent(False)
+    b.set_broadphase(1.25, 3, 0.35)
+    b.run_chunks(scs[0].dt, 10, 15)
+    h = list_lengths(b)
+    assert h.sum() == b.rbd_count()
+    assert h[-1] < b.rbd_count() // 50, f"bodies without a list: {h[-1]} of {h.sum()}"
+    src = [sc.build_physics()[0] for sc in scs]
+    b.writeback(src)
+    for w in (0, 5, 11):
+        alone = scs[w].build_physics()[0]
+        alone.step(scs[w].dt, 150)
+        for k in ("position", "position_old", "velocity"):
+            assert np.array_equal(bits(src[w].bodies()[k]), bits(alone.bodies()[k])), (w, k)
