@@ -966,7 +966,10 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
         g.n_seg = d.n_worlds;
         e = exclusive_scan_u32_segments(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream, g);
     } else {
-        e = exclusive_scan_u32(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream);
+        // one world: the cells its new grid uses and the end sentinel (its count read on the device)
+        const uint32_t* last =
+            reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(d.grids) + offsetof(GridParams, ncells));
+        e = exclusive_scan_u32(d.cell_count, d.cell_start, d.total_cells + 1, d.sort, d.stream, last);
     }
     if (e != hipSuccess) return e;
     SphereScatter k;

@@ -1105,10 +1105,10 @@ int World::upload() {
         if (W > 1) wid.resize(nb);
         for (uint32_t w = 0; w < W; ++w) {
             const uint32_t n_w = wstart[w + 1] - wstart[w];
-            // cells of the world's grid: worlds of a batch spread (nothing bounds them), and a grid
-            // that outgrows its cells doubles its cell size, which crowds the partner walks; a
-            // batch's tables are scanned only where a grid uses them, so room is cheap
-            const uint64_t cap = W > 1 ? std::max<uint64_t>(8ull * n_w, 8192ull) : std::max<uint64_t>(2ull * n_w, 1024ull);
+            // cells of the world's grid: worlds spread (nothing bounds them), and a grid that
+            // outgrows its cells doubles its cell size, which crowds the partner walks; the
+            // rebuild scans only the cells a grid uses, so room is cheap
+            const uint64_t cap = std::max<uint64_t>(8ull * n_w, 8192ull);
             const uint64_t base = (uint64_t)cell_base[w] + ((cap + 63ull) & ~63ull);
             if (base >= 0xFFFFFFF0ull) return fail(BG_ERR_CAPACITY, "cell table of the batch exceeds 2^32 entries");
             cell_base[w + 1] = (uint32_t)base;
@@ -1121,7 +1121,6 @@ int World::upload() {
                 }
             }
         }
-        if (W == 1) cell_base[1] = std::max<uint32_t>(next_pow2(std::max<uint32_t>(2u * nb, 1024u)), cell_base[1]);
         d.n_worlds = W;
         d.total_cells = cell_base[W];
         cell_rows = d.total_cells;
