@@ -372,3 +372,38 @@ def test_bodies_with_many_colliders_bitexact():
         assert_bitexact(p.bodies()["position"], o.pos, f"step {step} position")
     assert o.contact_count > n
     assert (p.bodies()["rotation"] == 0.0).all()
+
+
+def test_joint_block_beyond_the_lds_staging():
+    """A block of 64 joint components stages its joints and small components'
+    bodies in LDS when they fit (512 joints, 640 bodies).  Here a chain of 600
+    joints comes first, so the block that holds it and 40 small chains (1-15
+    joints) exceeds the staging and every component of it iterates in global
+    memory: positions bit-exact against the oracle."""
+    rng = np.random.default_rng(12)
+    pos, ja, jb = [], [], []
+    row = 0.0
+    for length in [600] + [1 + k % 15 for k in range(40)]:
+        base = len(pos)
+        pos += [(0.8 * k, row) for k in range(length + 1)]
+        ja += list(range(base, base + length))
+        jb += list(range(base + 1, base + length + 1))
+        row += 3.0
+    pos = np.array(pos, np.float32)
+    n = pos.shape[0]
+    vel = rng.normal(0, 2, (n, 2)).astype(np.float32)
+    rad = np.full(n, 0.3, np.float32)
+    sc = scenes._single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1 / 60, 3, {})
+    sc.joint_a = np.array(ja, np.int32)
+    sc.joint_b = np.array(jb, np.int32)
+    nj = sc.joint_a.shape[0]
+    sc.joint_anchor_a = rng.uniform(-0.2, 0.2, (nj, 2)).astype(np.float32)
+    sc.joint_anchor_b = rng.uniform(-0.2, 0.2, (nj, 2)).astype(np.float32)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    for step in range(3):
+        p.step(sc.dt)
+        o.step(sc.dt)
+        assert p.step_path() == "general"
+        assert_bitexact(p.bodies()["position"], o.pos, f"step {step} position")
+    assert np.abs(p.bodies()["position"] - pos).max(axis=1).min() > 0
