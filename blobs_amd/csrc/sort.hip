@@ -45,16 +45,11 @@ __device__ __forceinline__ uint32_t scan_len(uint32_t n, const uint32_t* last) {
     return m < n ? m : n;
 }
 
-// sums[tile] = sum of the tile's items (coalesced: item k * 256 + tid).
-__global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
-                                                            uint32_t* __restrict__ sums, const uint32_t* last) {
-    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
-    n = scan_len(n, last);
-    if (blockIdx.x * kScanTile >= n) {  // a tile past the items: nothing to read
-        if (threadIdx.x == 0) sums[blockIdx.x] = 0u;
-        return;
-    }
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x;
+// Sum of the items of the tile starting at tile0 (coalesced: item k * 256 + tid),
+// valid in thread 0.
+__device__ __forceinline__ uint32_t tile_sum(const uint32_t* __restrict__ in, uint32_t n, uint32_t tile0,
+                                             uint32_t* s_wave) {
+    const uint32_t base = tile0 + threadIdx.x;
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
@@ -64,12 +59,23 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __re
     sum = wave_sum_u32(sum);
     if ((threadIdx.x & (BG_WAVE - 1)) == 0) s_wave[threadIdx.x / BG_WAVE] = sum;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
+    uint32_t t = 0;
 #pragma unroll
-        for (int w = 0; w < kSortThreads / BG_WAVE; ++w) t += s_wave[w];
-        sums[blockIdx.x] = t;
+    for (int w = 0; w < kSortThreads / BG_WAVE; ++w) t += s_wave[w];
+    return t;
+}
+
+// sums[tile] = sum of the tile's items.
+__global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __restrict__ in, uint32_t n,
+                                                            uint32_t* __restrict__ sums, const uint32_t* last) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    n = scan_len(n, last);
+    if (blockIdx.x * kScanTile >= n) {  // a tile past the items: nothing to read
+        if (threadIdx.x == 0) sums[blockIdx.x] = 0u;
+        return;
     }
+    const uint32_t t = tile_sum(in, n, blockIdx.x * kScanTile, s_wave);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
 }
 
 // out = exclusive scan of in.  The tile goes through LDS (padded against
@@ -77,20 +83,9 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_sums(const uint32_t* __re
 // scans the 16 consecutive items [t*16, t*16+16).
 __device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
 
-__global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                             const uint32_t* __restrict__ sums, const uint32_t* last) {
-    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
-    __shared__ uint32_t s_t[kScanTile + kScanTile / 32];
-    n = scan_len(n, last);
-    const uint32_t tile0 = blockIdx.x * kScanTile;
-    if (tile0 >= n) return;  // the whole block leaves together (no barrier is skipped)
-    // this tile's offset: the sum of the earlier tiles' sums, reduced by the block itself
-    // (no separate pass over the tile sums)
-    uint32_t pre = 0;
-    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSortThreads) pre += sums[t];
-    uint32_t tile_off;
-    (void)block_excl_scan(pre, s_wave, &tile_off);
-    __syncthreads();  // s_wave is reused below
+// Exclusive scan of the tile starting at tile0, offset by tile_off.
+__device__ __forceinline__ void scan_tile(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t tile0,
+                                          uint32_t tile_off, uint32_t* s_wave, uint32_t* s_t) {
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t idx = k * kSortThreads + threadIdx.x;
@@ -120,6 +115,23 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in,
     }
 }
 
+__global__ __launch_bounds__(kSortThreads) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                             const uint32_t* __restrict__ sums, const uint32_t* last) {
+    __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
+    __shared__ uint32_t s_t[kScanTile + kScanTile / 32];
+    n = scan_len(n, last);
+    const uint32_t tile0 = blockIdx.x * kScanTile;
+    if (tile0 >= n) return;  // the whole block leaves together (no barrier is skipped)
+    // this tile's offset: the sum of the earlier tiles' sums, reduced by the block itself
+    // (no separate pass over the tile sums)
+    uint32_t pre = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSortThreads) pre += sums[t];
+    uint32_t tile_off;
+    (void)block_excl_scan(pre, s_wave, &tile_off);
+    __syncthreads();  // s_wave is reused below
+    scan_tile(in, out, n, tile0, tile_off, s_wave, s_t);
+}
+
 // Whether some segment uses a cell of tile t: only the segment holding the
 // tile's first cell can end before the tile; one starting inside it uses at
 // least its first cell.
@@ -138,22 +150,8 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_sums_seg(const uint32_t* 
         if (threadIdx.x == 0) sums[blockIdx.x] = 0u;
         return;
     }
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x;
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t i = base + k * kSortThreads;
-        sum += (i < n) ? in[i] : 0u;
-    }
-    sum = wave_sum_u32(sum);
-    if ((threadIdx.x & (BG_WAVE - 1)) == 0) s_wave[threadIdx.x / BG_WAVE] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < kSortThreads / BG_WAVE; ++w) t += s_wave[w];
-        sums[blockIdx.x] = t;
-    }
+    const uint32_t t = tile_sum(in, n, blockIdx.x * kScanTile, s_wave);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
 }
 
 // sums[0..n) -> their exclusive scan, in place, by one block (kScanTile sums a pass).
@@ -187,35 +185,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_tiles_seg(const uint32_t*
     __shared__ uint32_t s_wave[kSortThreads / BG_WAVE];
     __shared__ uint32_t s_t[kScanTile + kScanTile / 32];
     if (!tile_used(g, blockIdx.x)) return;  // the whole block leaves together
-    const uint32_t tile0 = blockIdx.x * kScanTile;
-    const uint32_t tile_off = offsets[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t idx = k * kSortThreads + threadIdx.x;
-        s_t[scan_pad(idx)] = (tile0 + idx < n) ? in[tile0 + idx] : 0u;
-    }
-    __syncthreads();
-    const uint32_t base = threadIdx.x * kSortItems;
-    uint32_t v[kSortItems];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        v[k] = s_t[scan_pad(base + k)];
-        sum += v[k];
-    }
-    uint32_t total;
-    uint32_t run = tile_off + block_excl_scan(sum, s_wave, &total);
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        s_t[scan_pad(base + k)] = run;
-        run += v[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        const uint32_t idx = k * kSortThreads + threadIdx.x;
-        if (tile0 + idx < n) out[tile0 + idx] = s_t[scan_pad(idx)];
-    }
+    scan_tile(in, out, n, blockIdx.x * kScanTile, offsets[blockIdx.x], s_wave, s_t);
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
