@@ -44,6 +44,8 @@ constexpr uint32_t kFlatMax = 128;  // candidates of a flat list (two 64-bit mas
 constexpr int kContacts = 4;      // contacts summed from registers; more take a selection pass in key order
 constexpr int kHitBatch = 2;      // marked candidates whose snapshot and mass load together
 constexpr int kSubstepWaves = 6;  // waves per SIMD the substep kernel is compiled for
+constexpr int kBatchWaves = 5;    // the same for batches (world lookups: at six it spills)
+constexpr int kGatherWaves = 4;   // the same for the first substep after a rebuild (no spills)
 constexpr int kSubstepThreads = 64;  // threads per block of the substep kernel (one wave)
 static_assert(kSubstepThreads >= 64 && kSubstepThreads % 64 == 0,
               "block_contacts holds one slot per 64 bodies: substep blocks are whole waves");
@@ -136,7 +138,7 @@ __device__ __forceinline__ void grid_rows(const GridParams& g, const uint32_t* _
 }
 
 template <bool GROUPS, bool SENSORS, bool ZERO_OFF, bool CONSTRAINTS, bool GATHER, bool BATCHED>
-__global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu(kSubstepWaves))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
+__global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu(GATHER ? kGatherWaves : (BATCHED ? kBatchWaves : kSubstepWaves)))) void k_sphere_substep(SphereSubstep k, SubstepArgs a) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kSubstepThreads + threadIdx.x;
     // clear the outlier table of the substep after next
