@@ -1127,7 +1127,7 @@ int World::upload() {
         HIP_TRY(dmalloc(&d.cell_base, W + 1));
         HIP_TRY(dmalloc(&d.grids, W));
         d.stat_rep = 1;
-        while (d.stat_rep < 256u && (size_t)d.stat_rep * 2u * W <= 4096u) d.stat_rep *= 2u;
+        while (d.stat_rep < 256u && (size_t)d.stat_rep * 2u * W <= 32768u) d.stat_rep *= 2u;
         HIP_TRY(dmalloc(&d.wbbox, 4u * d.stat_rep * (size_t)W));
         HIP_TRY(dmalloc(&d.wold_dt, W));
         HIP_TRY(dmalloc(&d.wstat, 8u * d.stat_rep * (size_t)W));
