@@ -271,6 +271,7 @@ struct SubstepArgs {
     float ratio_rest;   // h / h: displacement ratio of every non-static body but the first (physics.rs:338)
     float gx, gy;       // Physics::gravity
     float rmax;         // largest collider radius
+    float mass_per_r;   // sphere path: every body's mass is this times its radius (0: load the masses)
     float drift_prev_x, drift_prev_y;  // common drift at the previous snapshot (broadphase query centre)
     float drift_x, drift_y;            // common drift after this substep
     float skin;         // sphere path: displacement since the rebuild beyond which a body is an outlier

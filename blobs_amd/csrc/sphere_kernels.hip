@@ -189,6 +189,9 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             // same expression as the skin test that wrote it)
             const float2 vw = k.wvel[w];
             const float dpx = a.drift_prev_x + vw.x * a.t_prev, dpy = a.drift_prev_y + vw.y * a.t_prev;
+            // a partner's mass is mass_per_r times the radius in its snapshot (exact: a power of
+            // two times a normal value, checked for every body at upload)
+            const bool by_r = a.mass_per_r != 0.0f;
             // reference predicate (physics.rs:255-269) against the slot-t snapshot; the
             // partner's mass is `mp` when the caller loaded it (have_m), else loaded on a hit
             auto test = [&](uint32_t t, const float4 sb, float mp, bool have_m, auto&& sink) {
@@ -206,7 +209,8 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                 if (!(dist < rs)) return;
                 const uint32_t idb = __float_as_uint(sb.w) & ~kSnapOutlier;
                 const bool isA = id > idb;
-                const float mpart = have_m ? mp : k.src.mg[GATHER ? k.perm[t] : t].x;
+                const float mpart = have_m ? mp
+                                           : (by_r ? a.mass_per_r * sb.z : k.src.mg[GATHER ? k.perm[t] : t].x);
                 sink(t, idb, isA, isA ? dx : -dx, isA ? dy : -dy, dist, rs, mpart);
             };
             // Partner lists.  Two non-outliers are each within `skin` of their rebuild
@@ -266,6 +270,13 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             // candidates (flat walk or partner list) within the squared contact distance
             uint64_t near = 0, near_hi = 0;  // candidates 0..63 and 64..127
             uint64_t lst = 0, lst_hi = 0;    // flat candidates within the list reach (first substep after a rebuild)
+            // The first two candidates of the first eight list entries keep their snapshots
+            // (with masses from the radius, nothing else of the partner is needed), so a
+            // body with at most two takes no second round of loads.  Groups and sensors read
+            // the partner's other records and do not keep them.
+            constexpr bool kKeep = BATCHED && !GATHER && !GROUPS && !SENSORS;
+            float4 kept0 = make_float4(0.f, 0.f, 0.f, 0.f);
+            uint32_t n_kept = 0;
             if (flat) {
                 for (uint32_t f = 0; f < n_flat; f += 4) {
                     uint32_t tt[4];
@@ -313,6 +324,16 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                         const bool cand = u < nl && !(__float_as_uint(sb[u].w) & kSnapOutlier) &&
                                           !(dx * dx + dy * dy > (rs * rs) * 1.00000095367431640625f);
                         near |= cand ? (1ull << u) : 0ull;
+                        if (kKeep) {
+                            const bool kp = cand && by_r;
+                            // componentwise: a select of whole records would go through memory
+                            const bool k0 = kp && n_kept == 0u;
+                            kept0.x = k0 ? sb[u].x : kept0.x;
+                            kept0.y = k0 ? sb[u].y : kept0.y;
+                            kept0.z = k0 ? sb[u].z : kept0.z;
+                            kept0.w = k0 ? sb[u].w : kept0.w;
+                            n_kept += kp ? 1u : 0u;
+                        }
                     }
                 }
                 // the rest, four a round
@@ -370,8 +391,14 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             }
             auto enumerate = [&](auto&& sink) {
                 if (flat || use_list) {
-                    // marked candidates kHitBatch at a time: their snapshots and masses load together
                     uint64_t m = near, m2 = near_hi;
+                    if (kKeep && n_kept > 0u) {
+                        // the kept candidates are the lowest marked entries
+                        const uint32_t i0 = (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1;
+                        test(list_slot(i0), kept0, a.mass_per_r * kept0.z, true, sink);
+                    }
+                    // marked candidates kHitBatch at a time: their snapshots and masses load together
                     while ((m | m2) != 0) {
                         uint32_t tt[kHitBatch];
                         bool ok[kHitBatch];
@@ -391,11 +418,11 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
                         for (int u = 0; u < kHitBatch; ++u) {
                             sb[u] = k.snap_in[tt[u]];
-                            mp[u] = k.src.mg[GATHER ? k.perm[tt[u]] : tt[u]].x;
+                            mp[u] = by_r ? 0.0f : k.src.mg[GATHER ? k.perm[tt[u]] : tt[u]].x;
                         }
 #pragma unroll
                         for (int u = 0; u < kHitBatch; ++u)
-                            if (ok[u]) test(tt[u], sb[u], mp[u], true, sink);
+                            if (ok[u]) test(tt[u], sb[u], by_r ? a.mass_per_r * sb[u].z : mp[u], true, sink);
                     }
                 } else {
                     grid_rows(g, cs, k.nb, xa.x - dpx, xa.y - dpy, Qc,

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 
 namespace bg {
@@ -1207,6 +1208,26 @@ int World::upload() {
             rw[i] = make_float2(rot[i], omega[i]);
             mg[i] = make_float2(mass[i], gmod[i]);
         }
+        // A single-sphere body's calculated_mass is 2 r per push of its collider (4 r with
+        // the double push).  When every body's mass is exactly k r for one power of two k
+        // (normal values only, so the product is exact on host and device alike), the
+        // contact pass takes a partner's mass from the radius in its snapshot record
+        // instead of loading the partner's mass record.
+        mass_per_r_ = 0.0f;
+        for (const float kk : {4.0f, 2.0f}) {
+            bool all = nb > 0;
+            for (uint32_t i = 0; i < nb && all; ++i) {
+                const float m = kk * crad[i];
+                uint32_t mb, wb;
+                memcpy(&mb, &m, 4);
+                memcpy(&wb, &mass[i], 4);
+                all = std::isnormal(crad[i]) && std::isnormal(m) && mb == wb;
+            }
+            if (all) {
+                mass_per_r_ = kk;
+                break;
+            }
+        }
         HIP_TRY(h2d(d.s.pp, pp, s));
         HIP_TRY(h2d(d.s.snap, snap, s));
         HIP_TRY(h2d(d.s.vel, vel, s));
@@ -1588,6 +1609,7 @@ int World::run_substep(float h, bool fresh, bool last) {
     a.gx = params_.gravity[0];
     a.gy = params_.gravity[1];
     a.rmax = rmax_;
+    a.mass_per_r = path_ == StepPath::Sphere ? mass_per_r_ : 0.0f;
     a.drift_prev_x = (float)drift_x_;
     a.drift_prev_y = (float)drift_y_;
     if (use_drift_) {

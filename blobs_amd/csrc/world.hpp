@@ -185,6 +185,7 @@ private:
     uint32_t rebuild_interval_ = 0;  // 0: every step below 65,536 bodies, every third step from there
     // world properties fixed at upload
     float rmax_ = 0.0f;
+    float mass_per_r_ = 0.0f;  // sphere path: every body's mass is exactly this times its radius (0: not so)
     int32_t first_dyn_ = -1;
     bool groups_ = false, sensors_ = false, zero_off_ = false, use_drift_ = false;
     bool structure_error_ = false;
@@ -214,8 +215,9 @@ private:
     std::vector<uint32_t> batch_start_;
     // batch stepped by the persistent kernel (one workgroup per world): allowed, and in
     // effect since the last upload (single centred spheres, default groups, no sensors,
-    // constraints or events, worlds of at most kPersistMaxBodies bodies)
-    bool persist_enabled_ = true, persist_ = false;
+    // constraints or events, worlds of at most kPersistMaxBodies bodies).  Off by default:
+    // over long runs of spreading worlds the grid path alone is faster
+    bool persist_enabled_ = false, persist_ = false;
     SphereArrays p_src_{};  // where the last persistent launch (or the upload) left its worlds' records
     std::vector<float> batch_old_dt_;
     std::vector<double> batch_time_;

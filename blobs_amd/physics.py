@@ -492,8 +492,8 @@ class Physics:
 
     def set_persistent(self, enable: bool) -> None:
         """Step this batch with the persistent kernel (one workgroup per
-        world, one launch per step call or chunk) when it allows it; the
-        default.  Off: the grid path."""
+        world, one launch per step call or chunk) when it allows it.  Off
+        (the default): the grid path."""
         self._chk(self._lib.bg_batch_set_persistent(self._w, int(bool(enable))))
 
     def persistent_active(self) -> bool:

@@ -223,15 +223,15 @@ int bg_batch_writeback(bg_world *batch, bg_world *const *worlds, uint32_t n_worl
 /* Number of worlds in a batch (0 for a plain world). */
 int bg_batch_size(bg_world *w, uint32_t *out);
 /* Step the batch's worlds of 1,025 to 4,096 bodies with the persistent
- * kernel (default on): one launch per bg_step / chunk, one workgroup per
+ * kernel (default off): one launch per bg_step / chunk, one workgroup per
  * world taken from a largest-first queue, every substep of the launch inside
  * the workgroup with the world's snapshot, grid and state on chip; the other
  * worlds step on the grid path.  In effect from the next step when the
  * batch's colliders are centred with default groups and no sensors, and no
  * collision events are recorded; otherwise (or with enable = 0) every world
- * steps on the grid path.  Results are the same bits either way.
- * bg_autotune times both schedules and keeps the faster.  No counterpart in
- * the reference (it has no batches). */
+ * steps on the grid path.  Results are the same bits either way.  With it
+ * on, bg_autotune times both schedules and keeps the faster.  No counterpart
+ * in the reference (it has no batches). */
 int bg_batch_set_persistent(bg_world *w, int enable);
 /* *active = 1 if the batch's last upload gave worlds to the persistent kernel. */
 int bg_batch_persistent(bg_world *w, int *active);
@@ -302,8 +302,8 @@ int bg_debug_data(bg_world *w, bg_affine2 *bodies, bg_affine2 *colliders, float 
  * rebuild position before it is tracked as an outlier; fused sphere path)
  * and the number of steps between grid rebuilds (>= 1).  Results do not
  * depend on any of them; only speed does.  Until it is called: cell scale
- * 1.25, skin 0.25, a rebuild every step below 65,536 bodies and every third
- * step from there. */
+ * 1.25 (1.5 for a batch), skin 0.25, a rebuild every step below 65,536
+ * bodies and every third step from there. */
 int bg_set_broadphase(bg_world *w, float cell_scale, float skin, uint32_t rebuild_interval);
 /* Which device path the last step ran: 0 none, 1 general, 2 fused spheres. */
 int bg_step_path(bg_world *w, uint32_t *out);

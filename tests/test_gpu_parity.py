@@ -340,6 +340,41 @@ def test_crowded_bodies_beyond_the_contact_lists_bitexact():
     assert o.contact_count > 25 * n  # pairs: over 50 contacts per body on average
 
 
+@pytest.mark.parametrize("masses", ["default", "override_half_r", "override_r", "one_off"])
+def test_partner_masses_from_radius_or_loaded(masses):
+    """The sphere path takes a partner's mass from the radius in its
+    snapshot when every body's mass is exactly 2 r or 4 r (checked at
+    upload), and loads the mass records otherwise.  Crowded spheres with
+    forced overlaps, both ways, bit-exact against the oracle over 20 steps:
+    default masses (4 r), overrides of r / 2 on every collider (mass r after
+    the double push: neither 2 r nor 4 r), overrides of r (mass 2 r), and
+    default masses but for one body (loaded)."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    pos = rng.uniform(0.0, 60.0, (n, 2)).astype(np.float32)
+    vel = rng.normal(0.0, 1.0, (n, 2)).astype(np.float32)
+    rad = rng.uniform(0.3, 0.6, n).astype(np.float32)
+    sc = scenes._single_sphere_scene(pos, vel, rad, (0.0, -9.81), 1 / 60, 20, {})
+    if masses == "override_half_r":
+        sc.col_has_mo[:] = 1
+        sc.col_mass_override[:] = rad * np.float32(0.5)
+    elif masses == "override_r":
+        sc.col_has_mo[:] = 1
+        sc.col_mass_override[:] = rad
+    elif masses == "one_off":
+        sc.col_has_mo[17] = 1
+        sc.col_mass_override[17] = np.float32(3.0)
+    p = gpu_world(sc)
+    o = OracleWorld(sc, np.float32)
+    for step in range(20):
+        p.step(sc.dt)
+        o.step(sc.dt)
+        assert p.contact_count() == o.contact_count, f"step {step}: contact count"
+    assert p.step_path() == "sphere"
+    assert o.contact_count > n // 4
+    compare_state(p, o, masses)
+
+
 def test_bodies_with_many_colliders_bitexact():
     """Bodies of 12 colliders each (the general contact pass stages a body's
     first 8 colliders and reads the rest from memory).  Without joints or

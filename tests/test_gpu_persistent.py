@@ -48,8 +48,9 @@ def mixed_batch(seed=80):
 def test_persistent_equals_grid_path_and_worlds_alone():
     scs = mixed_batch()
     a = Physics.batch([sc.build_physics()[0] for sc in scs])
+    a.set_persistent(True)
     g = Physics.batch([sc.build_physics()[0] for sc in scs])
-    g.set_persistent(False)
+    assert not g.persistent_active()  # the default: the grid path
     a.step(scs[0].dt, 1)
     g.step(scs[0].dt, 1)
     assert a.persistent_active() and not g.persistent_active()
@@ -94,6 +95,7 @@ def test_persistent_user_inputs_static_bodies_and_q1():
     src = [p for p, _ in built[0]]
     ref = [p for p, _ in built[1]]
     batch = Physics.batch(src)
+    batch.set_persistent(True)
     batch.step(scs[0].dt, 6)
     assert batch.persistent_active()
     batch.writeback(src)
@@ -106,6 +108,7 @@ def test_persistent_user_inputs_static_bodies_and_q1():
 def test_persistent_off_with_events_and_on_again():
     scs = mixed_batch(seed=82)[:4]
     b = Physics.batch([sc.build_physics()[0] for sc in scs])
+    b.set_persistent(True)
     b.set_event_capacity(1 << 16)
     b.step(scs[0].dt, 2)
     assert not b.persistent_active()  # events need the grid path's per-substep record
@@ -126,6 +129,7 @@ def test_persistent_collisions_disabled():
         p.collisions_enabled = False
         ws.append(p)
     b = Physics.batch(ws)
+    b.set_persistent(True)
     b.step(scs[0].dt, 4)
     assert b.persistent_active() and b.contact_count() == 0
     for w, sc in enumerate(scs):
