@@ -272,6 +272,7 @@ struct SubstepArgs {
     float gx, gy;       // Physics::gravity
     float rmax;         // largest collider radius
     float mass_per_r;   // sphere path: every body's mass is this times its radius (0: load the masses)
+    uint32_t gmod_one;  // sphere path: every body's gravity_mod is exactly 1
     float drift_prev_x, drift_prev_y;  // common drift at the previous snapshot (broadphase query centre)
     float drift_x, drift_y;            // common drift after this substep
     float skin;         // sphere path: displacement since the rebuild beyond which a body is an outlier

@@ -162,7 +162,12 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
         const uint32_t wa = __float_as_uint(own.w);
         const uint32_t id = wa & ~kSnapOutlier;
         const bool was_out = (wa & kSnapOutlier) != 0u;
-        const float2 mgv = k.src.mg[g_own];
+        // With every mass exactly mass_per_r times the radius and every gravity_mod 1 (both
+        // checked at upload), a steady substep needs nothing from the mass record: the own
+        // mass is mass_per_r times the snapshot radius, and g * 1 is g exactly.  (Batches
+        // load it: measured 0.4% faster there.)
+        const bool no_mg = !GATHER && !BATCHED && a.mass_per_r != 0.0f && a.gmod_one != 0u;
+        const float2 mgv = no_mg ? make_float2(0.0f, 1.0f) : k.src.mg[g_own];
         // loaded now, used after the contact pass: their latency overlaps the candidate walk.
         // A body whose rotation cannot change in this substep skips it.
         const bool still = ZERO_OFF && !GATHER && !a.fresh && (fl & BF_ROT_STILL) != 0u;
@@ -175,7 +180,7 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
         uint32_t lw[4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) lw[j] = GATHER ? 0u : k.nbr[j * k.nb + s];
-        const float mown = mgv.x;
+        const float mown = no_mg ? a.mass_per_r * own.z : mgv.x;
         if (a.collisions && finitef(own.x) && finitef(own.y)) {
             const float2 xa = make_float2(own.x, own.y);
             const float ra = own.z;

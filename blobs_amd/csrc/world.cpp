@@ -1228,6 +1228,11 @@ int World::upload() {
                 break;
             }
         }
+        gmod_one_ = std::all_of(gmod.begin(), gmod.begin() + nb, [](float g) {
+            uint32_t gb;
+            memcpy(&gb, &g, 4);
+            return gb == 0x3F800000u;
+        });
         HIP_TRY(h2d(d.s.pp, pp, s));
         HIP_TRY(h2d(d.s.snap, snap, s));
         HIP_TRY(h2d(d.s.vel, vel, s));
@@ -1610,6 +1615,7 @@ int World::run_substep(float h, bool fresh, bool last) {
     a.gy = params_.gravity[1];
     a.rmax = rmax_;
     a.mass_per_r = path_ == StepPath::Sphere ? mass_per_r_ : 0.0f;
+    a.gmod_one = (path_ == StepPath::Sphere && gmod_one_) ? 1u : 0u;
     a.drift_prev_x = (float)drift_x_;
     a.drift_prev_y = (float)drift_y_;
     if (use_drift_) {

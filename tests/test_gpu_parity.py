@@ -340,15 +340,17 @@ def test_crowded_bodies_beyond_the_contact_lists_bitexact():
     assert o.contact_count > 25 * n  # pairs: over 50 contacts per body on average
 
 
-@pytest.mark.parametrize("masses", ["default", "override_half_r", "override_r", "one_off"])
+@pytest.mark.parametrize("masses", ["default", "override_half_r", "override_r", "one_off", "gravity_mod"])
 def test_partner_masses_from_radius_or_loaded(masses):
     """The sphere path takes a partner's mass from the radius in its
     snapshot when every body's mass is exactly 2 r or 4 r (checked at
     upload), and loads the mass records otherwise.  Crowded spheres with
     forced overlaps, both ways, bit-exact against the oracle over 20 steps:
     default masses (4 r), overrides of r / 2 on every collider (mass r after
-    the double push: neither 2 r nor 4 r), overrides of r (mass 2 r), and
-    default masses but for one body (loaded)."""
+    the double push: neither 2 r nor 4 r), overrides of r (mass 2 r),
+    default masses but for one body (loaded), and one body with a
+    gravity_mod of 0.5 (a steady substep skips the mass record only while
+    every mass follows the radius and every gravity_mod is 1)."""
     rng = np.random.default_rng(11)
     n = 3000
     pos = rng.uniform(0.0, 60.0, (n, 2)).astype(np.float32)
@@ -364,6 +366,8 @@ def test_partner_masses_from_radius_or_loaded(masses):
     elif masses == "one_off":
         sc.col_has_mo[17] = 1
         sc.col_mass_override[17] = np.float32(3.0)
+    elif masses == "gravity_mod":
+        sc.gravity_mod[5] = np.float32(0.5)
     p = gpu_world(sc)
     o = OracleWorld(sc, np.float32)
     for step in range(20):
