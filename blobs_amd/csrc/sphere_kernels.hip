@@ -47,6 +47,7 @@ constexpr int kSubstepWaves = 6;  // waves per SIMD the substep kernel is compil
 constexpr int kBatchWaves = 5;    // the same for batches (world lookups: at six it spills)
 constexpr int kGatherWaves = 4;   // the same for the first substep after a rebuild (no spills)
 constexpr int kSubstepThreads = 64;  // threads per block of the substep kernel (one wave)
+constexpr uint32_t kStatBlocks = 1024;  // blocks of a single world's statistics pass
 static_assert(kSubstepThreads >= 64 && kSubstepThreads % 64 == 0,
               "block_contacts holds one slot per 64 bodies: substep blocks are whole waves");
 
@@ -705,25 +706,34 @@ __global__ __launch_bounds__(kThreads) void k_world_stats(const float4* __restri
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = 0.0f;
     uint32_t b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu, b2 = 0u, b3 = 0u;  // f2ord bounding box
-    if (live) {
-        const float4 sn = snap[i];
+    auto add = [&](uint32_t j) {
+        const float4 sn = snap[j];
         if (finitef(sn.x) && finitef(sn.y)) {
-            b0 = b2 = f2ord(sn.x);
-            b1 = b3 = f2ord(sn.y);
-            if (!(bflags[i] & BF_STATIC)) {
-                const float2 vv = vel[i];
+            const uint32_t ox = f2ord(sn.x), oy = f2ord(sn.y);
+            b0 = min(b0, ox);
+            b2 = max(b2, ox);
+            b1 = min(b1, oy);
+            b3 = max(b3, oy);
+            if (!(bflags[j] & BF_STATIC)) {
+                const float2 vv = vel[j];
                 if (finitef(vv.x) && finitef(vv.y)) {
-                    v[0] = vv.x;
-                    v[1] = vv.y;
-                    v[2] = 1.0f;
+                    v[0] += vv.x;
+                    v[1] += vv.y;
+                    v[2] += 1.0f;
                 }
             }
-            v[3] = sn.x;
-            v[4] = sn.y;
-            v[5] = sn.x * sn.x;
-            v[6] = sn.y * sn.y;
-            v[7] = 1.0f;
+            v[3] += sn.x;
+            v[4] += sn.y;
+            v[5] += sn.x * sn.x;
+            v[6] += sn.y * sn.y;
+            v[7] += 1.0f;
         }
+    };
+    if (BATCHED) {
+        if (live) add(i);
+    } else {
+        // one world: a grid-stride pass (a few bodies a thread), so fewer blocks add atomics
+        for (uint32_t j = i; j < nb; j += gridDim.x * blockDim.x) add(j);
     }
     const int lane = threadIdx.x & (BG_WAVE - 1), wv = threadIdx.x / BG_WAVE;
     const uint32_t w0 = __shfl(w, 0, BG_WAVE);
@@ -975,8 +985,8 @@ hipError_t launch_sphere_rebuild(DeviceWorld& d, float cell, uint32_t parity) {
         k_world_stats<true><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
                                                                d.stat_rep, d.wbbox, d.p_skip);
     else
-        k_world_stats<false><<<blocks, kThreads, 0, d.stream>>>(d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat,
-                                                                d.stat_rep, d.wbbox, nullptr);
+        k_world_stats<false><<<std::min(blocks, kStatBlocks), kThreads, 0, d.stream>>>(
+            d.s.snap, d.s.vel, d.s.bflags, d.nb, d.wid, d.wstat, d.stat_rep, d.wbbox, nullptr);
     k_world_grids<<<d.n_worlds, BG_WAVE, 0, d.stream>>>(d.n_worlds, d.wbbox, cell,
                                                                            d.cell_base, d.wstat, d.stat_rep,
                                                                            d.grids, d.wvel, d.accum);
