@@ -265,15 +265,17 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             };
             const uint32_t nl = use_list ? nlen : 0u;
             // slot of list entry i (< kNbrCap): the first eight from registers
-            // Batches: list words 4 and 5 (entries 8-11) of a list longer than eight entries load
-            // with the first eight entries' snapshots, so their round skips the word load.
+            // Batches: list words 4 to 6 (entries 8-13) of a list longer than eight entries load
+            // with the first eight entries' snapshots, so their rounds skip the word load.
             constexpr bool kPre45 = BATCHED && !GATHER;
-            uint32_t lw4 = 0u, lw5 = 0u;
+            uint32_t lw4 = 0u, lw5 = 0u, lw6 = 0u;
             auto list_slot = [&](uint32_t i) {
                 const uint32_t j = i >> 1;
                 const uint32_t a0 = (j & 1u) ? lw[1] : lw[0], a1 = (j & 1u) ? lw[3] : lw[2];
                 const uint32_t wd = j < 4u ? ((j & 2u) ? a1 : a0)
-                                           : (kPre45 && j == 4u ? lw4 : (kPre45 && j == 5u ? lw5 : k.nbr[j * k.nb + s]));
+                                           : (kPre45 && j == 4u ? lw4
+                                              : (kPre45 && j == 5u ? lw5
+                                              : (kPre45 && j == 6u ? lw6 : k.nbr[j * k.nb + s])));
                 const int32_t dt = (int32_t)(int16_t)(uint16_t)((i & 1u) ? (wd >> 16) : (wd & 0xFFFFu));
                 const uint32_t t = (uint32_t)((int32_t)s + dt);
                 return t < k.nb ? t : k.nb - 1u;
@@ -331,6 +333,7 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                     if (kPre45) {
                         lw4 = nl > 8u ? k.nbr[4u * k.nb + s] : 0u;
                         lw5 = nl > 10u ? k.nbr[5u * k.nb + s] : 0u;
+                        lw6 = nl > 12u ? k.nbr[6u * k.nb + s] : 0u;
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < 8; ++u) {
