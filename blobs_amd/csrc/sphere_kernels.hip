@@ -264,18 +264,18 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                 return t;
             };
             const uint32_t nl = use_list ? nlen : 0u;
-            // slot of list entry i (< kNbrCap): the first eight from registers
             // Batches: list words 4 to 6 (entries 8-13) of a list longer than eight entries load
             // with the first eight entries' snapshots, so their rounds skip the word load.
-            constexpr bool kPre45 = BATCHED && !GATHER;
+            constexpr bool kPreWords = BATCHED && !GATHER;
             uint32_t lw4 = 0u, lw5 = 0u, lw6 = 0u;
+            // slot of list entry i (< kNbrCap): the first eight (batches: fourteen) from registers
             auto list_slot = [&](uint32_t i) {
                 const uint32_t j = i >> 1;
                 const uint32_t a0 = (j & 1u) ? lw[1] : lw[0], a1 = (j & 1u) ? lw[3] : lw[2];
                 const uint32_t wd = j < 4u ? ((j & 2u) ? a1 : a0)
-                                           : (kPre45 && j == 4u ? lw4
-                                              : (kPre45 && j == 5u ? lw5
-                                              : (kPre45 && j == 6u ? lw6 : k.nbr[j * k.nb + s])));
+                                           : (kPreWords && j == 4u ? lw4
+                                              : (kPreWords && j == 5u ? lw5
+                                              : (kPreWords && j == 6u ? lw6 : k.nbr[j * k.nb + s])));
                 const int32_t dt = (int32_t)(int16_t)(uint16_t)((i & 1u) ? (wd >> 16) : (wd & 0xFFFFu));
                 const uint32_t t = (uint32_t)((int32_t)s + dt);
                 return t < k.nb ? t : k.nb - 1u;
@@ -283,10 +283,11 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             // candidates (flat walk or partner list) within the squared contact distance
             uint64_t near = 0, near_hi = 0;  // candidates 0..63 and 64..127
             uint64_t lst = 0, lst_hi = 0;    // flat candidates within the list reach (first substep after a rebuild)
-            // The first two candidates of the first eight list entries keep their snapshots
-            // (with masses from the radius, nothing else of the partner is needed), so a
-            // body with at most two takes no second round of loads.  Groups and sensors read
-            // the partner's other records and do not keep them.
+            // Batches: the first candidate of the first eight list entries keeps its snapshot
+            // (with masses from the radius, nothing else of the partner is needed), so a body
+            // with at most one takes no second round of loads.  Groups and sensors read the
+            // partner's other records and keep none; the plain variant has no registers to
+            // spare for it.
             constexpr bool kKeep = BATCHED && !GATHER && !GROUPS && !SENSORS;
             float4 kept0 = make_float4(0.f, 0.f, 0.f, 0.f);
             uint32_t n_kept = 0;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                     for (uint32_t u = 0; u < 8; ++u) tt[u] = u < nl ? list_slot(u) : s;
 #pragma unroll
                     for (uint32_t u = 0; u < 8; ++u) sb[u] = k.snap_in[tt[u]];
-                    if (kPre45) {
+                    if (kPreWords) {
                         lw4 = nl > 8u ? k.nbr[4u * k.nb + s] : 0u;
                         lw5 = nl > 10u ? k.nbr[5u * k.nb + s] : 0u;
                         lw6 = nl > 12u ? k.nbr[6u * k.nb + s] : 0u;
@@ -416,7 +417,8 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                         m &= m - 1;
                         test(list_slot(i0), kept0, a.mass_per_r * kept0.z, true, sink);
                     }
-                    // marked candidates kHitBatch at a time: their snapshots and masses load together
+                    // marked candidates kHitBatch at a time: their snapshots (and masses, unless
+                    // they follow from the radius) load together
                     while ((m | m2) != 0) {
                         uint32_t tt[kHitBatch];
                         bool ok[kHitBatch];
