@@ -49,6 +49,12 @@ struct SphereArrays {
 };
 
 constexpr uint32_t kSnapOutlier = 0x80000000u;
+// A single world below 2^24 bodies also carries its body flags (bits 0-5) in bits 24-29 of
+// the snapshot word, so a steady substep reads them with the snapshot; its ids are then the
+// low 24 bits.
+constexpr uint32_t kSnapFlagShift = 24;
+constexpr uint32_t kSnapFlagMask = 0x3Fu;
+constexpr uint32_t kSnapIdMask = 0x00FFFFFFu;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
 constexpr uint32_t kNbrCap = 32;      // partner-list entries per body (sphere path)
 constexpr uint8_t kNbrNone = 0xFF;    // no list: the body walks the grid
@@ -273,6 +279,7 @@ struct SubstepArgs {
     float rmax;         // largest collider radius
     float mass_per_r;   // sphere path: every body's mass is this times its radius (0: load the masses)
     uint32_t gmod_one;  // sphere path: every body's gravity_mod is exactly 1
+    uint32_t snap_flags;  // sphere path: snapshot words carry the body flags (ids in the low 24 bits)
     float drift_prev_x, drift_prev_y;  // common drift at the previous snapshot (broadphase query centre)
     float drift_x, drift_y;            // common drift after this substep
     float skin;         // sphere path: displacement since the rebuild beyond which a body is an outlier

@@ -154,14 +154,18 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
     uint32_t new_outliers = 0, count = 0, pushes = 0, flags = 0;
     if (s < k.nb && !(BATCHED && k.skip && k.skip[w])) {
         const uint32_t g_own = GATHER ? k.perm[s] : s;
-        const uint32_t fl = k.src.bflags[g_own];
+        const float4 own = k.snap_in[s];
+        const uint32_t wa = __float_as_uint(own.w);
+        // A single world's snapshot words carry the body flags: a steady substep reads them
+        // there instead of loading the flag record.
+        const bool snap_fl = !BATCHED && a.snap_flags != 0u;
+        const uint32_t fl = (snap_fl && !GATHER) ? ((wa >> kSnapFlagShift) & kSnapFlagMask) : k.src.bflags[g_own];
         const float4 pp = k.src.pp[g_own];
         BodyState st;
         st.p = make_float2(pp.x, pp.y);
         st.po = make_float2(pp.z, pp.w);
-        const float4 own = k.snap_in[s];
-        const uint32_t wa = __float_as_uint(own.w);
-        const uint32_t id = wa & ~kSnapOutlier;
+        const uint32_t idm = snap_fl ? kSnapIdMask : ~kSnapOutlier;  // id bits of a snapshot word
+        const uint32_t id = wa & idm;
         const bool was_out = (wa & kSnapOutlier) != 0u;
         // With every mass exactly mass_per_r times the radius and every gravity_mod 1 (both
         // checked at upload), a steady substep needs nothing from the mass record: the own
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                 if (dot > (rs * rs) * 1.00000095367431640625f) return;
                 const float dist = sqrtf(dot);
                 if (!(dist < rs)) return;
-                const uint32_t idb = __float_as_uint(sb.w) & ~kSnapOutlier;
+                const uint32_t idb = __float_as_uint(sb.w) & idm;
                 const bool isA = id > idb;
                 const float mpart = have_m ? mp
                                            : (by_r ? a.mass_per_r * sb.z : k.src.mg[GATHER ? k.perm[t] : t].x);
@@ -650,7 +654,8 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             out = !(ex * ex + ey * ey <= a.skin * a.skin);
             new_outliers += out ? 1u : 0u;
         }
-        k.snap_out[s] = make_float4(x.x, x.y, own.z, __uint_as_float(id | (out ? kSnapOutlier : 0u)));
+        const uint32_t fl_bits = snap_fl ? (fl_out & kSnapFlagMask) << kSnapFlagShift : 0u;
+        k.snap_out[s] = make_float4(x.x, x.y, own.z, __uint_as_float(id | fl_bits | (out ? kSnapOutlier : 0u)));
         if (out && finite_x) {
             // outlier table of the next substep
             const GridParams g = k.grids[w];

@@ -1199,9 +1199,10 @@ int World::upload() {
         d.gather_pending = false;
         std::vector<float4> pp(nb), snap(nb);
         std::vector<float2> rw(nb), mg(nb);
+        snap_flags_ = W == 1 && nb <= kSnapIdMask;
         for (uint32_t i = 0; i < nb; ++i) {
             pp[i] = make_float4(pos[i].x, pos[i].y, pos_old[i].x, pos_old[i].y);
-            uint32_t idbits = i;
+            uint32_t idbits = i | (snap_flags_ ? (bflags[i] & kSnapFlagMask) << kSnapFlagShift : 0u);
             float w;
             memcpy(&w, &idbits, 4);
             snap[i] = make_float4(cabs[i].x, cabs[i].y, crad[i], w);
@@ -1515,7 +1516,7 @@ int World::download() {
             cabs[k] = make_float2(snap[k].x, snap[k].y);
             uint32_t w;
             memcpy(&w, &snap[k].w, 4);
-            aid[k] = w & ~kSnapOutlier;
+            aid[k] = w & (snap_flags_ ? kSnapIdMask : ~kSnapOutlier);
         }
     } else {
         HIP_TRY(d2h(pos, d.b.pos, nb, s));
@@ -1616,6 +1617,7 @@ int World::run_substep(float h, bool fresh, bool last) {
     a.rmax = rmax_;
     a.mass_per_r = path_ == StepPath::Sphere ? mass_per_r_ : 0.0f;
     a.gmod_one = (path_ == StepPath::Sphere && gmod_one_) ? 1u : 0u;
+    a.snap_flags = (path_ == StepPath::Sphere && snap_flags_) ? 1u : 0u;
     a.drift_prev_x = (float)drift_x_;
     a.drift_prev_y = (float)drift_y_;
     if (use_drift_) {

@@ -187,6 +187,7 @@ private:
     float rmax_ = 0.0f;
     float mass_per_r_ = 0.0f;  // sphere path: every body's mass is exactly this times its radius (0: not so)
     bool gmod_one_ = false;    // sphere path: every body's gravity_mod is exactly 1
+    bool snap_flags_ = false;  // sphere path: snapshot words carry the body flags (one world, < 2^24 bodies)
     int32_t first_dyn_ = -1;
     bool groups_ = false, sensors_ = false, zero_off_ = false, use_drift_ = false;
     bool structure_error_ = false;
