@@ -287,14 +287,6 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             // candidates (flat walk or partner list) within the squared contact distance
             uint64_t near = 0, near_hi = 0;  // candidates 0..63 and 64..127
             uint64_t lst = 0, lst_hi = 0;    // flat candidates within the list reach (first substep after a rebuild)
-            // Batches: the first candidate of the first eight list entries keeps its snapshot
-            // (with masses from the radius, nothing else of the partner is needed), so a body
-            // with at most one takes no second round of loads.  Groups and sensors read the
-            // partner's other records and keep none; the plain variant has no registers to
-            // spare for it.
-            constexpr bool kKeep = BATCHED && !GATHER && !GROUPS && !SENSORS;
-            float4 kept0 = make_float4(0.f, 0.f, 0.f, 0.f);
-            uint32_t n_kept = 0;
             if (flat) {
                 for (uint32_t f = 0; f < n_flat; f += 4) {
                     uint32_t tt[4];
@@ -347,16 +339,6 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                         const bool cand = u < nl && !(__float_as_uint(sb[u].w) & kSnapOutlier) &&
                                           !(dx * dx + dy * dy > (rs * rs) * 1.00000095367431640625f);
                         near |= cand ? (1ull << u) : 0ull;
-                        if (kKeep) {
-                            const bool kp = cand && by_r;
-                            // componentwise: a select of whole records would go through memory
-                            const bool k0 = kp && n_kept == 0u;
-                            kept0.x = k0 ? sb[u].x : kept0.x;
-                            kept0.y = k0 ? sb[u].y : kept0.y;
-                            kept0.z = k0 ? sb[u].z : kept0.z;
-                            kept0.w = k0 ? sb[u].w : kept0.w;
-                            n_kept += kp ? 1u : 0u;
-                        }
                     }
                 }
                 // the rest, four a round
@@ -415,12 +397,6 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
             auto enumerate = [&](auto&& sink) {
                 if (flat || use_list) {
                     uint64_t m = near, m2 = near_hi;
-                    if (kKeep && n_kept > 0u) {
-                        // the kept candidates are the lowest marked entries
-                        const uint32_t i0 = (uint32_t)__builtin_ctzll(m);
-                        m &= m - 1;
-                        test(list_slot(i0), kept0, a.mass_per_r * kept0.z, true, sink);
-                    }
                     // marked candidates kHitBatch at a time: their snapshots (and masses, unless
                     // they follow from the radius) load together
                     while ((m | m2) != 0) {
