@@ -268,18 +268,21 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                 return t;
             };
             const uint32_t nl = use_list ? nlen : 0u;
-            // Batches: list words 4 to 6 (entries 8-13) of a list longer than eight entries load
+            // Batches: list words 4 to 9 (entries 8-19) of a list longer than eight entries load
             // with the first eight entries' snapshots, so their rounds skip the word load.
             constexpr bool kPreWords = BATCHED && !GATHER;
-            uint32_t lw4 = 0u, lw5 = 0u, lw6 = 0u;
-            // slot of list entry i (< kNbrCap): the first eight (batches: fourteen) from registers
+            uint32_t lw4 = 0u, lw5 = 0u, lw6 = 0u, lw7 = 0u, lw8 = 0u, lw9 = 0u;
+            // slot of list entry i (< kNbrCap): the first eight (batches: twenty) from registers
             auto list_slot = [&](uint32_t i) {
                 const uint32_t j = i >> 1;
                 const uint32_t a0 = (j & 1u) ? lw[1] : lw[0], a1 = (j & 1u) ? lw[3] : lw[2];
                 const uint32_t wd = j < 4u ? ((j & 2u) ? a1 : a0)
                                            : (kPreWords && j == 4u ? lw4
                                               : (kPreWords && j == 5u ? lw5
-                                              : (kPreWords && j == 6u ? lw6 : k.nbr[j * k.nb + s])));
+                                              : (kPreWords && j == 6u ? lw6
+                                              : (kPreWords && j == 7u ? lw7
+                                              : (kPreWords && j == 8u ? lw8
+                                              : (kPreWords && j == 9u ? lw9 : k.nbr[j * k.nb + s]))))));
                 const int32_t dt = (int32_t)(int16_t)(uint16_t)((i & 1u) ? (wd >> 16) : (wd & 0xFFFFu));
                 const uint32_t t = (uint32_t)((int32_t)s + dt);
                 return t < k.nb ? t : k.nb - 1u;
@@ -331,6 +334,9 @@ __global__ __launch_bounds__(kSubstepThreads) __attribute__((amdgpu_waves_per_eu
                         lw4 = nl > 8u ? k.nbr[4u * k.nb + s] : 0u;
                         lw5 = nl > 10u ? k.nbr[5u * k.nb + s] : 0u;
                         lw6 = nl > 12u ? k.nbr[6u * k.nb + s] : 0u;
+                        lw7 = nl > 14u ? k.nbr[7u * k.nb + s] : 0u;
+                        lw8 = nl > 16u ? k.nbr[8u * k.nb + s] : 0u;
+                        lw9 = nl > 18u ? k.nbr[9u * k.nb + s] : 0u;
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < 8; ++u) {
